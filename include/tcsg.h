@@ -1,0 +1,169 @@
+/*
+ * tcsg.h -- C ABI of the MI355X thread-coarsened stencil runtime (libtcsg.so).
+ *
+ * This is the drop-in boundary for the reference's generated launch
+ * interface.  The reference's host harness reaches its kernels through four
+ * C++ functions declared in hip/launchUpdateGrid.h:13-18 and emitted by
+ * hip/generateLaunchUpdateGrid.py; each entry point below names the one it
+ * replaces.  Differences that are deliberate:
+ *   - plain C linkage, status codes instead of exit()/throw, no hip headers
+ *     (streams and device pointers are void*);
+ *   - a loaded kernel is a handle (tcsg_kernel) owning its code object and its
+ *     mask buffer, so several stencils can live in one process (the
+ *     reference keeps one __constant__ mask per process);
+ *   - one-step launches take an explicit stream and do not synchronise.
+ *
+ * Kernels are code objects written by the generator
+ * (threadcoarsenedstencilgenerator_amd/generator) and compiled for gfx950.
+ */
+#ifndef TCSG_H
+#define TCSG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- enumerations (values match the generator's spec module) ---- */
+enum { TCSG_BOX = 0, TCSG_STAR = 1 };
+enum { TCSG_ROW = 0, TCSG_COLUMN = 1 };
+enum { TCSG_JACOBI = 0, TCSG_VARIABLE = 1 };
+enum { TCSG_ALIGN_NONE = 0, TCSG_ALIGN_READ = 1, TCSG_ALIGN_WRITE = 2 };
+enum { TCSG_F64 = 0, TCSG_F32C = 1 };
+enum { TCSG_PLAIN = 0, TCSG_STREAM = 1 };
+
+/* initial-value modes for tcsg_launch_init */
+enum {
+    TCSG_INIT_REFERENCE = 0, /* the reference's init kernel values        */
+    TCSG_INIT_U01 = 1,       /* interior U[0,1), halo 0 (SplitMix64)       */
+    TCSG_INIT_UPM1 = 2,      /* interior U[-1,1), halo 0                   */
+    TCSG_INIT_NORMAL = 3     /* interior N(0,1) by Box-Muller, halo 0      */
+};
+
+/* ---- status codes ---- */
+enum {
+    TCSG_OK = 0,
+    TCSG_ERR_ARG = 1,          /* invalid argument                                   */
+    TCSG_ERR_HIP = 2,          /* a HIP runtime call failed (see tcsg_last_hip_error) */
+    TCSG_ERR_MISMATCH = 3,     /* code object parameters differ from the descriptor  */
+    TCSG_ERR_BLOCK_SHAPE = 4,  /* block shape not launchable for this kernel         */
+    TCSG_ERR_TOO_LARGE = 5,    /* grid does not fit this kernel's addressing         */
+    TCSG_ERR_NO_MASK = 6,      /* Variable kernel launched before tcsg_set_mask      */
+    TCSG_ERR_FILE = 7          /* code object missing or unreadable                  */
+};
+
+typedef struct tcsg_dim3 { uint32_t x, y, z; } tcsg_dim3;
+
+/* The generator's compile-time parameters (the reference generator argv,
+ * hip/generate2DStencil_hip.py:28-44, hip/generate3DStencil_hip.py:29-46,
+ * plus the element type and code shape). */
+typedef struct tcsg_desc {
+    uint32_t dimension;      /* 2 or 3                      */
+    uint32_t neighbourhood;  /* TCSG_BOX / TCSG_STAR        */
+    uint32_t orientation;    /* TCSG_ROW / TCSG_COLUMN      */
+    uint32_t coeff;          /* TCSG_JACOBI / TCSG_VARIABLE */
+    uint32_t range;          /* r >= 1                      */
+    uint32_t xc, yc, zc;     /* coarsening (zc = 1 in 2D)   */
+    uint32_t alignment;      /* TCSG_ALIGN_*                */
+    uint32_t cl_size;        /* alignment size in points    */
+    uint32_t dtype;          /* TCSG_F64 / TCSG_F32C        */
+    uint32_t variant;        /* TCSG_PLAIN / TCSG_STREAM    */
+} tcsg_desc;
+
+/* Padded geometry of a grid (hip/benchmark.cpp:140-177, hip/benchmark.h:100-123). */
+typedef struct tcsg_geometry {
+    tcsg_dim3 size;       /* allocation extents incl. halo and padding (the reference's gridSize) */
+    tcsg_dim3 elements;   /* interior points per axis (gridElements)  */
+    tcsg_dim3 threads;    /* threads per axis (gridThreads)           */
+    uint32_t left_offset; /* padding before each row (Write alignment) */
+    uint32_t right_offset;/* padding after each row                    */
+    uint32_t dimension;
+    uint32_t range;
+    uint64_t n_points;    /* size.x * size.y * size.z (64-bit)         */
+} tcsg_geometry;
+
+typedef struct tcsg_kernel tcsg_kernel;
+
+/* Geometry for a user grid size incl. halo (`size`, z ignored in 2D).
+ * Replaces the geometry part of stencilBenchmark::initBenchmark
+ * (hip/benchmark.cpp:133-182). */
+int tcsg_geometry_compute(const tcsg_desc *desc, tcsg_dim3 size, tcsg_geometry *out);
+
+/* Load a generated code object on the current device and check its baked-in
+ * parameters against *expect (NULL: accept).  Replaces the compile-time
+ * binding of kernels.cu into launchUpdateGrid.cu
+ * (hip/generateLaunchUpdateGrid.py:55-58) and its sanity check (:106-119). */
+int tcsg_kernel_load(tcsg_kernel **out, const char *code_object_path, const tcsg_desc *expect);
+int tcsg_kernel_free(tcsg_kernel *k);
+int tcsg_kernel_desc(const tcsg_kernel *k, tcsg_desc *out);
+
+/* Variable-mask upload (n doubles in the layout of the reference's d_mask).
+ * Replaces copyMask (hip/launchUpdateGrid.h:18, hip/generateLaunchUpdateGrid.py:77-82).
+ * F32C kernels receive the FP32 rounding of each weight. */
+int tcsg_set_mask(tcsg_kernel *k, const double *mask, uint32_t n);
+
+/* Block-size limits.  occupancy_optimal is what the reference's
+ * maxPotentialBlockSize returns (hip/launchUpdateGrid.h:17,
+ * hip/generateLaunchUpdateGrid.py:84-89); legal_max is the largest block the
+ * kernel can be launched with.  Either pointer may be NULL. */
+int tcsg_max_block_size(tcsg_kernel *k, int *occupancy_optimal, int *legal_max);
+
+/* 0 if `block` can be launched with this kernel, else TCSG_ERR_BLOCK_SHAPE. */
+int tcsg_check_block(const tcsg_kernel *k, tcsg_dim3 block);
+
+/* Blocks per axis = ceil(threads / block) (hip/benchmark.cpp:232,301-303). */
+tcsg_dim3 tcsg_grid_blocks(const tcsg_geometry *g, tcsg_dim3 block);
+
+/* Fill a whole grid buffer (elements of the kernel's dtype) with initial
+ * values.  TCSG_INIT_REFERENCE replaces launchInit
+ * (hip/launchUpdateGrid.h:14, hip/generateLaunchUpdateGrid.py:150-158); the
+ * seeded modes are counter-based on the interior index.  Padding is zeroed.
+ * Asynchronous on `stream`. */
+int tcsg_launch_init(tcsg_kernel *k, const tcsg_geometry *g, int mode, uint64_t seed,
+                     void *d_array, void *stream);
+
+/* One time step: out = stencil(in) on the interior (halo and padding of
+ * `out` untouched).  The launch in the reference's benchmark loop
+ * (hip/generateLaunchUpdateGrid.py:92-100,131).  Asynchronous. */
+int tcsg_update_grid(tcsg_kernel *k, const tcsg_geometry *g, tcsg_dim3 block, tcsg_dim3 blocks,
+                     const void *in, void *out, void *stream);
+
+/* `steps` ping-pong steps (step t reads buffer t%2), asynchronous.  *final
+ * receives the index (0/1) of the buffer holding the result. */
+int tcsg_run(tcsg_kernel *k, const tcsg_geometry *g, tcsg_dim3 block, tcsg_dim3 blocks,
+             void *d0, void *d1, int steps, void *stream, int *final_index);
+
+/* The reference's timing protocol (hip/generateLaunchUpdateGrid.py:103-147):
+ * doubling rounds of 1,2,4,... steps with a device synchronisation after
+ * every step and a host pointer swap, until the elapsed time exceeds
+ * max_time; returns the 2^k-1 per-step durations [s] in durations (up to
+ * cap; *n gets the full count).  Null stream. */
+int tcsg_benchmark_update_grid(tcsg_kernel *k, const tcsg_geometry *g, tcsg_dim3 block,
+                               tcsg_dim3 blocks, double max_time, void *d0, void *d1,
+                               double *durations, size_t cap, size_t *n);
+
+/* Event timing of `steps` back-to-back steps after `warmup` untimed ones,
+ * on `stream`, no host synchronisation between steps.  *ms = total time. */
+int tcsg_time_steps(tcsg_kernel *k, const tcsg_geometry *g, tcsg_dim3 block, tcsg_dim3 blocks,
+                    void *d0, void *d1, int warmup, int steps, void *stream, float *ms);
+
+/* Error of an F32C state against an FP64 state over the interior, computed
+ * in FP64 on the GPU: out[0] = max |a-b|, out[1] = max |b|,
+ * out[2] = sum (a-b)^2, out[3] = sum b^2, out[4] = max |a-b|/max(|b|, floor).
+ * Synchronous. */
+int tcsg_error_f32_vs_f64(const tcsg_geometry *g, const float *a, const double *b,
+                          double floor, double *out);
+
+/* FP64 -> FP32 copy of a whole grid (round to nearest).  Asynchronous. */
+int tcsg_convert_f64_to_f32(const double *src, float *dst, uint64_t n, void *stream);
+
+/* Readable name of a status code; last HIP error code seen by the library. */
+const char *tcsg_status_string(int status);
+int tcsg_last_hip_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TCSG_H */

@@ -1,0 +1,48 @@
+"""Shared GPU-vs-oracle runner for the parity tests."""
+from __future__ import annotations
+
+import numpy as np
+
+import oracle
+
+
+def run_case(desc, size, steps, block, mode=None, seed=1, mask_seed=3):
+    """Run `steps` steps of `desc` on the GPU and on the oracle from the same
+    initial state; returns (gpu_final, oracle_final, initial, geometry)."""
+    import torch
+
+    from threadcoarsenedstencilgenerator_amd import runtime as rt
+    from threadcoarsenedstencilgenerator_amd import spec
+
+    mode = rt.INIT_U01 if mode is None else mode
+    k = rt.Kernel(desc)
+    g = k.geometry(size)
+    mask = None
+    if desc.coeff == spec.VARIABLE:
+        mask = rt.mask_seeded(mask_seed, desc.n)
+        k.set_mask(mask)
+    grids = rt.Grids(g, k.dtype)
+    for i in range(2):
+        k.init(g, grids.ptr(i), mode, seed)
+    torch.cuda.synchronize()
+    a0 = grids.host(0)
+    fin = k.run(g, block, rt.grid_blocks(g, block), grids.ptr(0), grids.ptr(1), steps)
+    torch.cuda.synchronize()
+    got = grids.host(fin)
+    og = oracle.Geometry(desc.dimension, desc.range, desc.alignment, desc.cl_size, size, desc.coarsening)
+    assert (og.xdim, og.ydim, og.zdim, og.lo, og.ro) == (g.size.x, g.size.y, g.size.z, g.left_offset,
+                                                          g.right_offset)
+    if desc.dtype == spec.F32C:
+        ref = oracle.run_f32c(desc, og, mask, a0.copy(), a0.copy(), steps)
+    else:
+        ref = oracle.run_f64(desc, og, mask, a0.copy(), a0.copy(), steps)
+    return got, ref, a0, og
+
+
+def assert_bitwise(got, ref, what=""):
+    ui = np.uint32 if got.dtype == np.float32 else np.uint64
+    same = got.view(ui) == ref.view(ui)
+    if not same.all():
+        idx = np.flatnonzero(~same)
+        raise AssertionError(f"{what}: {idx.size} of {got.size} points differ; first at {idx[:5]}: "
+                             f"gpu {got.flat[idx[0]]!r} oracle {ref.flat[idx[0]]!r}")

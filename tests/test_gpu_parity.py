@@ -1,0 +1,45 @@
+"""Generated kernels vs the CPU oracle, bit for bit, over the option matrix
+(2D/3D x Box/Star x Row/Column x Jacobi/Variable x range/coarsening/
+alignment) in the three code shapes (plain, stream, stream+DPP), on small
+grids with truncated and padded geometry and a few time steps."""
+import pytest
+
+from threadcoarsenedstencilgenerator_amd import manifest, spec
+
+from .parity import assert_bitwise, run_case
+
+pytestmark = pytest.mark.gpu
+
+CASES = manifest.parity_cases()
+
+
+def _id(case):
+    d, size = case
+    return d.key()
+
+
+def _blocks(d):
+    if d.dimension == 3:
+        return [(8, 4, 2), (64, 1, 2)]
+    return [(16, 4, 1), (64, 2, 1)]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[_id(c) for c in CASES])
+def test_parity_f64(case):
+    d, size = case
+    for block in _blocks(d):
+        print(f"case {d.key()} size {size} block {block}", flush=True)
+        got, ref, _, _ = run_case(d, size, steps=3, block=block)
+        assert_bitwise(got, ref, f"{d.key()} block {block}")
+
+
+F32 = manifest.f32c_cases()
+
+
+@pytest.mark.parametrize("case", F32, ids=[_id(c) for c in F32])
+def test_parity_f32c(case):
+    d, size = case
+    block = (32, 2, 2) if d.dimension == 3 else (32, 4, 1)
+    print(f"case {d.key()} size {size} block {block}", flush=True)
+    got, ref, _, _ = run_case(d, size, steps=3, block=block, mode=2)
+    assert_bitwise(got, ref, d.key())

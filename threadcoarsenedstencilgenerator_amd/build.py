@@ -1,0 +1,87 @@
+"""Builds the native parts in-tree: the runtime library (libtcsg.so), the
+`bench` harness binary and generated kernel code objects (kernels/*.co).
+
+Every HIP compile targets gfx950 only, with -ffp-contract=off (the stencil
+arithmetic is written with explicit FMAs) and packed FP32 disabled.
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import subprocess
+
+from . import spec
+from .generator.emit import emit_source
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+INCLUDE = os.path.join(ROOT, "include")
+KDIR = os.path.join(PKG, "kernels")
+LIB = os.path.join(PKG, "libtcsg.so")
+BENCH_BIN = os.path.join(PKG, "harness", "bench")
+SHIM_LIB = os.path.join(PKG, "harness", "liblaunchUpdateGrid.so")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+HIP_FLAGS = [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-ffp-contract=off",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"command failed ({r.returncode}): {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return r
+
+
+def build_runtime() -> str:
+    src = os.path.join(PKG, "csrc", "tcsg_runtime.hip")
+    _run([HIPCC, *HIP_FLAGS, "-fPIC", "-shared", "-Wall", f"-I{INCLUDE}", src, "-o", LIB])
+    return LIB
+
+
+def kernel_path(desc: spec.StencilDesc) -> str:
+    return os.path.join(KDIR, desc.key() + ".co")
+
+
+def build_kernel(desc: spec.StencilDesc, force: bool = True) -> str:
+    """Emit and compile one kernel; returns the code-object path."""
+    os.makedirs(KDIR, exist_ok=True)
+    co = kernel_path(desc)
+    src = co[:-3] + ".hip"
+    text = emit_source(desc)
+    if not force and os.path.exists(co) and os.path.exists(src) and open(src).read() == text:
+        return co
+    tmp_src = src + f".{os.getpid()}.tmp"
+    with open(tmp_src, "w") as f:
+        f.write(text)
+    os.replace(tmp_src, src)
+    tmp = co + f".{os.getpid()}.tmp"
+    _run([HIPCC, *HIP_FLAGS, "--genco", "--no-gpu-bundle-output", "-x", "hip", src, "-o", tmp])
+    os.replace(tmp, co)
+    return co
+
+
+def build_kernels(descs, force: bool = True, jobs: int | None = None):
+    """Compile many kernels in parallel; returns {desc: path}."""
+    descs = list(dict.fromkeys(descs))
+    jobs = jobs or min(8, os.cpu_count() or 1)
+    out = {}
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        futs = {ex.submit(build_kernel, d, force): d for d in descs}
+        for f in cf.as_completed(futs):
+            out[futs[f]] = f.result()
+    return out
+
+
+def build_harness() -> str:
+    """The `bench` CLI (same argv as the reference's bench) and the
+    launchUpdateGrid.h compatibility library."""
+    hd = os.path.join(PKG, "harness")
+    srcs = [os.path.join(hd, n) for n in ("main.cpp", "benchmark.cpp", "primes.cpp")]
+    _run([HIPCC, "-O2", "-std=c++17", f"--offload-arch={ARCH}", f"-I{INCLUDE}", f"-I{hd}", *srcs,
+          "-o", BENCH_BIN, f"-L{PKG}", "-ltcsg", f"-Wl,-rpath,{PKG}", "-Wl,-rpath,$ORIGIN/.."])
+    _run([HIPCC, "-O2", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC", "-shared", f"-I{INCLUDE}",
+          os.path.join(hd, "launchUpdateGrid.cpp"), "-o", SHIM_LIB, f"-L{PKG}", "-ltcsg",
+          "-Wl,-rpath,$ORIGIN/.."])
+    return BENCH_BIN

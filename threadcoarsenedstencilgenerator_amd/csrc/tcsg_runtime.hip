@@ -1,0 +1,540 @@
+// libtcsg.so -- runtime behind include/tcsg.h.
+//
+// Loads generated gfx950 code objects, owns their mask buffers, computes the
+// padded geometry, launches steps and times them.  The device kernels in this
+// file are the ones every stencil shares: grid initialisation, FP64->FP32
+// conversion and the FP64 error reduction used by the FP32-storage variant.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <vector>
+
+#include "tcsg.h"
+
+namespace {
+
+thread_local int g_last_hip = 0;
+
+#define TCSG_HIP(call)                                  \
+    do {                                                \
+        hipError_t e_ = (call);                         \
+        if (e_ != hipSuccess) {                         \
+            g_last_hip = (int)e_;                       \
+            return TCSG_ERR_HIP;                        \
+        }                                               \
+    } while (0)
+
+constexpr uint32_t kMagic = 0x54435347u;
+constexpr uint32_t kAbi = 1u;
+// Offset given to out-of-interior lanes by STREAM kernels (the emitter's OOB_BASE).
+constexpr uint64_t kOobBase = 0xC0000000ull;
+
+// Kernel argument block; layout matches the emitted `struct tcsg_kargs`.
+struct KArgs {
+    const void *in;
+    void *out;
+    const void *mask;
+    uint32_t xdim, ydim, zdim, right_offset;
+    uint32_t in_bytes, out_bytes, pad0, pad1;
+};
+static_assert(sizeof(KArgs) == 56, "kernel argument layout");
+
+uint32_t stencil_size(const tcsg_desc &d)
+{
+    uint32_t w = 2 * d.range + 1;
+    if (d.neighbourhood == TCSG_BOX) return d.dimension == 2 ? w * w : w * w * w;
+    return 2 * d.dimension * d.range + 1;
+}
+
+size_t elem_size(const tcsg_desc &d) { return d.dtype == TCSG_F32C ? 4 : 8; }
+
+bool desc_valid(const tcsg_desc &d)
+{
+    if (d.dimension != 2 && d.dimension != 3) return false;
+    if (d.neighbourhood > 1 || d.orientation > 1 || d.coeff > 1 || d.dtype > 1 || d.variant > 1) return false;
+    if (d.range < 1 || d.xc < 1 || d.yc < 1 || d.zc < 1) return false;
+    if (d.dimension == 2 && d.zc != 1) return false;
+    if (d.alignment > 2) return false;
+    if (d.alignment != TCSG_ALIGN_NONE && d.cl_size == 0) return false;
+    return true;
+}
+
+uint32_t left_offset(const tcsg_desc &d)
+{
+    if (d.alignment == TCSG_ALIGN_WRITE && d.range % d.cl_size) return d.cl_size - d.range % d.cl_size;
+    return 0;
+}
+
+// ------------------------------------------------------------------ devices
+__device__ __forceinline__ uint64_t sm64(uint64_t seed, uint64_t idx)
+{
+    uint64_t z = seed + (idx + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ double u01(uint64_t seed, uint64_t idx)
+{
+    return (double)(sm64(seed, idx) >> 11) * 0x1.0p-53;
+}
+
+// Initial values over the whole padded grid.  Reference mode follows the
+// generated init kernels (hip/generate2DStencil_hip.py:322-351,
+// hip/generate3DStencil_hip.py:387-431); padding, which they leave unwritten,
+// is set to 0 here.
+template <typename T>
+__global__ void init_kernel(T *a, uint32_t xdim, uint32_t ydim, uint32_t zdim, uint32_t lo, uint32_t ro,
+                            uint32_t r, uint32_t dim, int mode, uint64_t seed)
+{
+    const uint64_t n = (uint64_t)xdim * ydim * zdim;
+    const uint64_t nx = xdim - lo - ro - 2 * r, ny = ydim - 2 * r;
+    const uint32_t zr = dim == 3 ? r : 0;
+    for (uint64_t idx = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; idx < n;
+         idx += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t i = (uint32_t)(idx % xdim);
+        const uint64_t jk = idx / xdim;
+        const uint32_t j = (uint32_t)(jk % ydim);
+        const uint32_t k = (uint32_t)(jk / ydim);
+        double v = 0.0;
+        if (i >= lo && i < xdim - ro) {
+            if (mode == TCSG_INIT_REFERENCE) {
+                const bool halo = (i - lo < r) || (j < r) || (dim == 3 && k < r);
+                if (halo) v = dim == 2 ? 1.0 : ((idx % 2 == 0) ? 0.75 : 0.25);
+            } else {
+                const bool inside = i >= lo + r && i < xdim - ro - r && j >= r && j < ydim - r &&
+                                    k >= zr && k < zdim - zr;
+                if (inside) {
+                    const uint64_t li = (i - lo - r) + nx * ((j - r) + ny * (uint64_t)(k - zr));
+                    if (mode == TCSG_INIT_U01) v = u01(seed, li);
+                    else if (mode == TCSG_INIT_UPM1) v = (double)(sm64(seed, li) >> 11) * 0x1.0p-52 - 1.0;
+                    else {
+                        const double a1 = 1.0 - u01(seed, 2 * li), a2 = u01(seed, 2 * li + 1);
+                        v = sqrt(-2.0 * log(a1)) * cos(6.283185307179586 * a2);
+                    }
+                }
+            }
+        }
+        a[idx] = (T)v;
+    }
+}
+
+__global__ void convert_kernel(const double *__restrict__ s, float *__restrict__ d, uint64_t n)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        d[i] = (float)s[i];
+}
+
+__device__ __forceinline__ void atomic_max_nonneg(double *p, double v)
+{
+    // non-negative doubles order like their bit patterns
+    atomicMax((unsigned long long *)p, (unsigned long long)__double_as_longlong(v));
+}
+
+// FP64 error reduction of an FP32 state against an FP64 state over the interior.
+__global__ void error_kernel(const float *__restrict__ a, const double *__restrict__ b, uint32_t xdim,
+                             uint32_t ydim, uint32_t zdim, uint32_t lo, uint32_t ro, uint32_t r,
+                             uint32_t dim, double floor_v, double *out)
+{
+    const uint32_t zr = dim == 3 ? r : 0;
+    const uint64_t nx = xdim - lo - ro - 2 * r, ny = ydim - 2 * r, nz = zdim - 2 * zr;
+    const uint64_t n = nx * ny * nz;
+    double mx = 0.0, mb = 0.0, s2 = 0.0, b2 = 0.0, mrel = 0.0;
+    for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < n;
+         t += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t i = t % nx, jk = t / nx, j = jk % ny, k = jk / ny;
+        const uint64_t idx = (lo + r + i) + xdim * ((r + j) + (uint64_t)ydim * (zr + k));
+        const double bv = b[idx];
+        const double d = fabs((double)a[idx] - bv);
+        mx = fmax(mx, d);
+        mb = fmax(mb, fabs(bv));
+        s2 += d * d;
+        b2 += bv * bv;
+        mrel = fmax(mrel, d / fmax(fabs(bv), floor_v));
+    }
+    // wave reduction then one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) {
+        mx = fmax(mx, __shfl_down(mx, off));
+        mb = fmax(mb, __shfl_down(mb, off));
+        s2 += __shfl_down(s2, off);
+        b2 += __shfl_down(b2, off);
+        mrel = fmax(mrel, __shfl_down(mrel, off));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomic_max_nonneg(&out[0], mx);
+        atomic_max_nonneg(&out[1], mb);
+        atomicAdd(&out[2], s2);
+        atomicAdd(&out[3], b2);
+        atomic_max_nonneg(&out[4], mrel);
+    }
+}
+
+int geometry_compute(const tcsg_desc &d, tcsg_dim3 size, tcsg_geometry &g)
+{
+    const uint32_t r = d.range;
+    uint32_t sz[3] = {size.x, size.y, d.dimension == 3 ? size.z : 1u};
+    const uint32_t c[3] = {d.xc, d.yc, d.zc};
+    uint32_t t[3] = {1, 1, 1}, e[3] = {1, 1, 1};
+    for (uint32_t a = 0; a < d.dimension; ++a) {
+        if (sz[a] <= 2 * r) return TCSG_ERR_ARG;
+        t[a] = sz[a] - (sz[a] - 2 * r) % c[a];
+        e[a] = t[a] - 2 * r;
+    }
+    uint64_t xmem = t[0], lo = 0, ro = 0;
+    if (d.alignment == TCSG_ALIGN_READ) {
+        if (xmem % d.cl_size) { ro = d.cl_size - xmem % d.cl_size; xmem += ro; }
+    } else if (d.alignment == TCSG_ALIGN_WRITE) {
+        lo = left_offset(d);
+        xmem += lo;
+        if (xmem % d.cl_size) ro = d.cl_size - xmem % d.cl_size;
+        xmem += ro;
+    }
+    if (xmem > 0xFFFFFFFFull) return TCSG_ERR_TOO_LARGE;
+    g.size = {(uint32_t)xmem, t[1], d.dimension == 3 ? t[2] : 1u};
+    g.elements = {e[0], e[1], d.dimension == 3 ? e[2] : 1u};
+    g.threads = {e[0] / c[0], e[1] / c[1], d.dimension == 3 ? e[2] / c[2] : 1u};
+    g.left_offset = (uint32_t)lo;
+    g.right_offset = (uint32_t)ro;
+    g.dimension = d.dimension;
+    g.range = r;
+    g.n_points = (uint64_t)g.size.x * g.size.y * g.size.z;
+    return TCSG_OK;
+}
+
+} // namespace
+
+struct tcsg_kernel {
+    hipModule_t module = nullptr;
+    hipFunction_t fn = nullptr;
+    tcsg_desc desc{};
+    uint32_t params[24]{};
+    void *mask = nullptr;
+    bool mask_set = false;
+    int legal_max = 0;
+    int xext_dpp = 0;     // STREAM kernel exchanging x-neighbours by DPP (params[16])
+};
+
+extern "C" {
+
+const char *tcsg_status_string(int s)
+{
+    switch (s) {
+    case TCSG_OK: return "ok";
+    case TCSG_ERR_ARG: return "invalid argument";
+    case TCSG_ERR_HIP: return "HIP runtime error";
+    case TCSG_ERR_MISMATCH: return "code object parameters differ from the descriptor";
+    case TCSG_ERR_BLOCK_SHAPE: return "block shape not launchable for this kernel";
+    case TCSG_ERR_TOO_LARGE: return "grid too large for this kernel's addressing";
+    case TCSG_ERR_NO_MASK: return "Variable kernel launched before its mask was set";
+    case TCSG_ERR_FILE: return "code object missing or unreadable";
+    default: return "unknown status";
+    }
+}
+
+int tcsg_last_hip_error(void) { return g_last_hip; }
+
+int tcsg_geometry_compute(const tcsg_desc *desc, tcsg_dim3 size, tcsg_geometry *out)
+{
+    if (!desc || !out || !desc_valid(*desc)) return TCSG_ERR_ARG;
+    return geometry_compute(*desc, size, *out);
+}
+
+tcsg_dim3 tcsg_grid_blocks(const tcsg_geometry *g, tcsg_dim3 b)
+{
+    tcsg_dim3 r{0, 0, 0};
+    if (!g || !b.x || !b.y || !b.z) return r;
+    r.x = (g->threads.x + b.x - 1) / b.x;
+    r.y = (g->threads.y + b.y - 1) / b.y;
+    r.z = (g->threads.z + b.z - 1) / b.z;
+    return r;
+}
+
+int tcsg_kernel_load(tcsg_kernel **out, const char *path, const tcsg_desc *expect)
+{
+    if (!out || !path) return TCSG_ERR_ARG;
+    *out = nullptr;
+    std::ifstream f(path, std::ios::binary);
+    if (!f) return TCSG_ERR_FILE;
+    std::vector<char> bytes((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+    if (bytes.empty()) return TCSG_ERR_FILE;
+    tcsg_kernel *k = new tcsg_kernel();
+    hipError_t e = hipModuleLoadData(&k->module, bytes.data());
+    if (e != hipSuccess) { g_last_hip = (int)e; delete k; return TCSG_ERR_HIP; }
+    auto fail = [&](int code) { (void)hipModuleUnload(k->module); delete k; return code; };
+    e = hipModuleGetFunction(&k->fn, k->module, "tcsg_update");
+    if (e != hipSuccess) { g_last_hip = (int)e; return fail(TCSG_ERR_HIP); }
+    hipDeviceptr_t p = nullptr;
+    size_t bytes_p = 0;
+    e = hipModuleGetGlobal(&p, &bytes_p, k->module, "tcsg_params");
+    if (e != hipSuccess || bytes_p != sizeof(k->params)) { g_last_hip = (int)e; return fail(TCSG_ERR_HIP); }
+    e = hipMemcpyDtoH(k->params, p, sizeof(k->params));
+    if (e != hipSuccess) { g_last_hip = (int)e; return fail(TCSG_ERR_HIP); }
+    const uint32_t *q = k->params;
+    if (q[0] != kMagic || q[1] != kAbi) return fail(TCSG_ERR_MISMATCH);
+    tcsg_desc d{q[2], q[3], q[4], q[5], q[6], q[7], q[8], q[9], q[10], q[11], q[12], q[13]};
+    if (!desc_valid(d) || q[14] != stencil_size(d)) return fail(TCSG_ERR_MISMATCH);
+    if (expect && std::memcmp(expect, &d, sizeof(d)) != 0) return fail(TCSG_ERR_MISMATCH);
+    k->desc = d;
+    int v = 0;
+    e = hipFuncGetAttribute(&v, HIP_FUNC_ATTRIBUTE_MAX_THREADS_PER_BLOCK, k->fn);
+    if (e != hipSuccess) { g_last_hip = (int)e; return fail(TCSG_ERR_HIP); }
+    k->legal_max = std::min<int>(v, (int)q[15]);
+    k->xext_dpp = d.variant == TCSG_STREAM && q[16] != 0;
+    *out = k;
+    return TCSG_OK;
+}
+
+int tcsg_kernel_free(tcsg_kernel *k)
+{
+    if (!k) return TCSG_OK;
+    if (k->mask) (void)hipFree(k->mask);
+    if (k->module) (void)hipModuleUnload(k->module);
+    delete k;
+    return TCSG_OK;
+}
+
+int tcsg_kernel_desc(const tcsg_kernel *k, tcsg_desc *out)
+{
+    if (!k || !out) return TCSG_ERR_ARG;
+    *out = k->desc;
+    return TCSG_OK;
+}
+
+int tcsg_set_mask(tcsg_kernel *k, const double *mask, uint32_t n)
+{
+    if (!k || !mask) return TCSG_ERR_ARG;
+    if (n != stencil_size(k->desc)) return TCSG_ERR_ARG;
+    const size_t es = elem_size(k->desc);
+    if (!k->mask) TCSG_HIP(hipMalloc(&k->mask, n * es));
+    if (es == 8) {
+        TCSG_HIP(hipMemcpy(k->mask, mask, n * 8, hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> f(n);
+        for (uint32_t i = 0; i < n; ++i) f[i] = (float)mask[i];
+        TCSG_HIP(hipMemcpy(k->mask, f.data(), n * 4, hipMemcpyHostToDevice));
+    }
+    k->mask_set = true;
+    return TCSG_OK;
+}
+
+int tcsg_max_block_size(tcsg_kernel *k, int *occ, int *legal)
+{
+    if (!k) return TCSG_ERR_ARG;
+    if (occ) {
+        int min_grid = 0, block = 0;
+        TCSG_HIP(hipModuleOccupancyMaxPotentialBlockSize(&min_grid, &block, k->fn, 0, 0));
+        *occ = block;
+    }
+    if (legal) *legal = k->legal_max;
+    return TCSG_OK;
+}
+
+int tcsg_check_block(const tcsg_kernel *k, tcsg_dim3 b)
+{
+    if (!k) return TCSG_ERR_ARG;
+    if (!b.x || !b.y || !b.z || b.x > 1024 || b.y > 1024 || b.z > 1024) return TCSG_ERR_BLOCK_SHAPE;
+    const uint64_t t = (uint64_t)b.x * b.y * b.z;
+    if (t > (uint64_t)k->legal_max) return TCSG_ERR_BLOCK_SHAPE;
+    if (k->xext_dpp) {
+        // every run of lanes in one block row and one wave must span the
+        // H = ceil(r/xc) lanes the x-neighbour exchange reaches
+        const uint32_t H = (k->desc.range + k->desc.xc - 1) / k->desc.xc;
+        for (uint64_t lid = 0; lid < t; ++lid) {
+            const uint32_t tx = (uint32_t)(lid % b.x), lane = (uint32_t)(lid & 63);
+            const uint32_t tl = std::min(tx, lane);
+            uint32_t tr = std::min(b.x - 1 - tx, 63u - lane);
+            // a partial last wave ends at the block's last thread
+            tr = (uint32_t)std::min<uint64_t>(tr, t - 1 - lid);
+            if (tl + tr + 1 < H) return TCSG_ERR_BLOCK_SHAPE;
+        }
+    }
+    return TCSG_OK;
+}
+
+static int check_geom(const tcsg_kernel *k, const tcsg_geometry *g)
+{
+    const tcsg_desc &d = k->desc;
+    if (g->dimension != d.dimension || g->range != d.range) return TCSG_ERR_ARG;
+    tcsg_geometry ref;
+    // the geometry must be one this kernel's coarsening and alignment produce
+    tcsg_dim3 user{g->size.x - g->left_offset - g->right_offset, g->size.y, g->size.z};
+    if (geometry_compute(d, user, ref) != TCSG_OK) return TCSG_ERR_ARG;
+    if (std::memcmp(&ref, g, sizeof(ref)) != 0) return TCSG_ERR_ARG;
+    if (d.variant == TCSG_STREAM) {
+        const uint64_t es = elem_size(d);
+        const uint64_t sx = (uint64_t)g->size.x * es, sxy = sx * g->size.y;
+        const uint64_t H = (d.range + d.xc - 1) / d.xc;
+        const uint64_t span = (d.zc + 2ull * d.range) * (d.dimension == 3 ? sxy : 0) +
+                              (d.yc + 2ull * d.range) * sx + (2 * H * d.xc + d.xc + 2ull * d.range) * es;
+        const uint64_t bytes = g->n_points * es;
+        if (bytes + span > kOobBase || span >= (1ull << 32) - kOobBase) return TCSG_ERR_TOO_LARGE;
+    }
+    return TCSG_OK;
+}
+
+int tcsg_launch_init(tcsg_kernel *k, const tcsg_geometry *g, int mode, uint64_t seed, void *a, void *stream)
+{
+    if (!k || !g || !a || mode < 0 || mode > 3) return TCSG_ERR_ARG;
+    const hipStream_t s = (hipStream_t)stream;
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>((g->n_points + 255) / 256, 65536);
+    if (k->desc.dtype == TCSG_F32C)
+        hipLaunchKernelGGL(init_kernel<float>, dim3(blocks), dim3(256), 0, s, (float *)a, g->size.x, g->size.y,
+                           g->size.z, g->left_offset, g->right_offset, g->range, g->dimension, mode, seed);
+    else
+        hipLaunchKernelGGL(init_kernel<double>, dim3(blocks), dim3(256), 0, s, (double *)a, g->size.x,
+                           g->size.y, g->size.z, g->left_offset, g->right_offset, g->range, g->dimension, mode,
+                           seed);
+    TCSG_HIP(hipGetLastError());
+    return TCSG_OK;
+}
+
+static int launch_one(tcsg_kernel *k, const tcsg_geometry *g, tcsg_dim3 block, tcsg_dim3 blocks, const void *in,
+                      void *out, hipStream_t s)
+{
+    KArgs a{};
+    a.in = in;
+    a.out = out;
+    a.mask = k->mask;
+    a.xdim = g->size.x;
+    a.ydim = g->size.y;
+    a.zdim = g->size.z;
+    a.right_offset = g->right_offset;
+    const uint64_t bytes = g->n_points * elem_size(k->desc);
+    a.in_bytes = a.out_bytes = (uint32_t)std::min<uint64_t>(bytes, 0xFFFFFFFFull);
+    size_t sz = sizeof(a);
+    void *extra[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, &a, HIP_LAUNCH_PARAM_BUFFER_SIZE, &sz, HIP_LAUNCH_PARAM_END};
+    TCSG_HIP(hipModuleLaunchKernel(k->fn, blocks.x, blocks.y, blocks.z, block.x, block.y, block.z, 0, s, nullptr,
+                                   extra));
+    return TCSG_OK;
+}
+
+static int precheck(tcsg_kernel *k, const tcsg_geometry *g, tcsg_dim3 block, tcsg_dim3 blocks)
+{
+    if (!k || !g) return TCSG_ERR_ARG;
+    int rc = check_geom(k, g);
+    if (rc) return rc;
+    rc = tcsg_check_block(k, block);
+    if (rc) return rc;
+    if (!blocks.x || !blocks.y || !blocks.z) return TCSG_ERR_ARG;
+    if ((uint64_t)blocks.x * block.x < g->threads.x || (uint64_t)blocks.y * block.y < g->threads.y ||
+        (uint64_t)blocks.z * block.z < g->threads.z)
+        return TCSG_ERR_ARG;  // grid would not cover the interior
+    if (k->desc.coeff == TCSG_VARIABLE && !k->mask_set) return TCSG_ERR_NO_MASK;
+    return TCSG_OK;
+}
+
+int tcsg_update_grid(tcsg_kernel *k, const tcsg_geometry *g, tcsg_dim3 block, tcsg_dim3 blocks, const void *in,
+                     void *out, void *stream)
+{
+    int rc = precheck(k, g, block, blocks);
+    if (rc) return rc;
+    if (!in || !out || in == out) return TCSG_ERR_ARG;
+    return launch_one(k, g, block, blocks, in, out, (hipStream_t)stream);
+}
+
+int tcsg_run(tcsg_kernel *k, const tcsg_geometry *g, tcsg_dim3 block, tcsg_dim3 blocks, void *d0, void *d1,
+             int steps, void *stream, int *final_index)
+{
+    int rc = precheck(k, g, block, blocks);
+    if (rc) return rc;
+    if (!d0 || !d1 || d0 == d1 || steps < 0) return TCSG_ERR_ARG;
+    void *b[2] = {d0, d1};
+    for (int s = 0; s < steps; ++s) {
+        rc = launch_one(k, g, block, blocks, b[s & 1], b[(s + 1) & 1], (hipStream_t)stream);
+        if (rc) return rc;
+    }
+    if (final_index) *final_index = steps & 1;
+    return TCSG_OK;
+}
+
+int tcsg_benchmark_update_grid(tcsg_kernel *k, const tcsg_geometry *g, tcsg_dim3 block, tcsg_dim3 blocks,
+                               double max_time, void *d0, void *d1, double *durations, size_t cap, size_t *n)
+{
+    int rc = precheck(k, g, block, blocks);
+    if (rc) return rc;
+    if (!d0 || !d1 || d0 == d1 || max_time < 0) return TCSG_ERR_ARG;
+    using clk = std::chrono::steady_clock;
+    std::vector<clk::time_point> ts;
+    ts.reserve(1024);
+    ts.push_back(clk::now());
+    double runtime = 0.0;
+    void *a0 = d0, *a1 = d1;
+    for (unsigned long long it = 1; runtime <= max_time; it *= 2) {
+        for (unsigned long long i = 0; i < it; ++i) {
+            rc = launch_one(k, g, block, blocks, a0, a1, nullptr);
+            if (rc) return rc;
+            TCSG_HIP(hipDeviceSynchronize());
+            std::swap(a0, a1);
+            ts.push_back(clk::now());
+        }
+        runtime = std::chrono::duration<double>(ts.back() - ts.front()).count();
+    }
+    const size_t cnt = ts.size() - 1;
+    for (size_t i = 0; i < cnt && i < cap; ++i)
+        durations[i] = std::chrono::duration<double>(ts[i + 1] - ts[i]).count();
+    if (n) *n = cnt;
+    return TCSG_OK;
+}
+
+int tcsg_time_steps(tcsg_kernel *k, const tcsg_geometry *g, tcsg_dim3 block, tcsg_dim3 blocks, void *d0,
+                    void *d1, int warmup, int steps, void *stream, float *ms)
+{
+    int rc = precheck(k, g, block, blocks);
+    if (rc) return rc;
+    if (!d0 || !d1 || d0 == d1 || warmup < 0 || steps < 1 || !ms) return TCSG_ERR_ARG;
+    const hipStream_t s = (hipStream_t)stream;
+    void *b[2] = {d0, d1};
+    int step = 0;
+    for (int w = 0; w < warmup; ++w, ++step) {
+        rc = launch_one(k, g, block, blocks, b[step & 1], b[(step + 1) & 1], s);
+        if (rc) return rc;
+    }
+    hipEvent_t e0, e1;
+    TCSG_HIP(hipEventCreate(&e0));
+    TCSG_HIP(hipEventCreate(&e1));
+    TCSG_HIP(hipEventRecord(e0, s));
+    for (int i = 0; i < steps; ++i, ++step) {
+        rc = launch_one(k, g, block, blocks, b[step & 1], b[(step + 1) & 1], s);
+        if (rc) { (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); return rc; }
+    }
+    TCSG_HIP(hipEventRecord(e1, s));
+    TCSG_HIP(hipEventSynchronize(e1));
+    TCSG_HIP(hipEventElapsedTime(ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return TCSG_OK;
+}
+
+int tcsg_error_f32_vs_f64(const tcsg_geometry *g, const float *a, const double *b, double floor_v, double *out)
+{
+    if (!g || !a || !b || !out) return TCSG_ERR_ARG;
+    double *d = nullptr;
+    TCSG_HIP(hipMalloc(&d, 5 * sizeof(double)));
+    hipError_t e = hipMemset(d, 0, 5 * sizeof(double));
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(error_kernel, dim3(1024), dim3(256), 0, nullptr, a, b, g->size.x, g->size.y, g->size.z,
+                           g->left_offset, g->right_offset, g->range, g->dimension, floor_v, d);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(out, d, 5 * sizeof(double), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) { g_last_hip = (int)e; return TCSG_ERR_HIP; }
+    return TCSG_OK;
+}
+
+int tcsg_convert_f64_to_f32(const double *src, float *dst, uint64_t n, void *stream)
+{
+    if (!src || !dst) return TCSG_ERR_ARG;
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(convert_kernel, dim3(blocks ? blocks : 1), dim3(256), 0, (hipStream_t)stream, src, dst, n);
+    TCSG_HIP(hipGetLastError());
+    return TCSG_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,446 @@
+"""HIP (gfx950) source emitter for one thread-coarsened stencil kernel.
+
+One call of :func:`emit_source` produces a translation unit with
+
+* ``tcsg_update`` -- one time step over the interior (the role of the
+  reference's generated ``updateGrid``, hip/generate2DStencil_hip.py:228-319,
+  hip/generate3DStencil_hip.py:242-384), taking a ``tcsg_kargs`` struct;
+* ``tcsg_params`` -- the compile-time parameters, which the runtime compares
+  with the caller's descriptor before any launch (the reference's compiled-in
+  sanity check, hip/generateLaunchUpdateGrid.py:106-119).
+
+Arithmetic contract, shared by every variant:  each output point sums its
+neighbourhood in the reference's traversal order (Row (dz,dy,dx), Column
+(dz,dx,dy), both lexicographic), Variable as ``acc = v0*w0`` then
+``acc = fma(v, w, acc)``, Jacobi as ``acc = v0``, ``acc = acc + v``,
+``out = acc * fl(1/n)``.  FMAs are explicit and the file is compiled with
+``-ffp-contract=off``, so the result does not depend on compiler contraction.
+
+Variants
+--------
+PLAIN   the reference's code shape: each thread owns an xc*yc*zc block,
+        loads every footprint point once into a register (global loads,
+        64-bit addressing), accumulates in traversal order, writes back.
+STREAM  the MI355X shape of the same computation:
+        * branch-free: buffer-resource loads/stores with 32-bit byte offsets
+          (loads of lanes outside the interior return 0, their stores are
+          dropped by the range check), so the compiler can pipeline the
+          loads of the next plane across the stores of the previous one;
+        * planes are consumed front to back and every output is stored as
+          soon as its last contributor has been added (z-marching register
+          window of 2r+1 planes);
+        * ``xext="dpp"``: x-neighbours outside the thread's columns come from
+          neighbouring lanes through wave64 DPP ``wave_shr:1``/``wave_shl:1``
+          lane shifts.  A run of lanes in one block row and one wave (a
+          segment) needs r columns beyond each of its ends; the H = ceil(r/xc)
+          lanes at each end load those with one extra load per row, and a
+          second DPP chain hands them to the lanes that need them, so no lane
+          loads an x-neighbour individually;
+        * optional XCD-aware block remapping (consecutive tiles on one XCD's
+          L2) and nontemporal stores.
+"""
+from __future__ import annotations
+
+from collections import defaultdict
+
+from .. import spec
+from ..spec import (ALIGN_NONE, BOX, COLUMN, F32C, F64, JACOBI, PLAIN, ROW, STAR,
+                    STREAM, VARIABLE, StencilDesc)
+
+# Byte offset given to lanes outside the interior.  The runtime only launches a
+# STREAM kernel when buffer bytes <= OOB_BASE - (largest negative footprint
+# delta) and OOB_BASE + (largest positive delta) < 2**32, so every access of
+# such a lane is past the buffer's record count and every access of a valid
+# lane is inside it.
+OOB_BASE = 0xC0000000
+
+DPP_WAVE_SHL1 = 0x130  # lane i receives lane i+1
+DPP_WAVE_SHR1 = 0x138  # lane i receives lane i-1
+
+
+def _n(v: int) -> str:
+    """Identifier-safe signed integer."""
+    return f"m{-v}" if v < 0 else str(v)
+
+
+def _hexf(x: float, f32: bool) -> str:
+    if f32:
+        import struct
+        xf = struct.unpack("f", struct.pack("f", x))[0]
+        return float(xf).hex() + "f"
+    return float(x).hex()
+
+
+class _Emitter:
+    def __init__(self, d: StencilDesc):
+        self.d = d
+        self.f32 = d.dtype == F32C
+        self.T = "float" if self.f32 else "double"
+        self.R = d.range
+        self.XC, self.YC, self.ZC = d.xc, d.yc, d.zc
+        self.three = d.dimension == 3
+        self.offs = spec.offsets(d)
+        self.tindex = {o: t for t, o in enumerate(self.offs)}
+        self.L = []
+
+    # ------------------------------------------------------------------ util
+    def w(self, s: str = ""):
+        self.L.append(s)
+
+    def outputs(self):
+        zr = range(self.ZC) if self.three else range(1)
+        return [(i, j, k) for k in zr for j in range(self.YC) for i in range(self.XC)]
+
+    def footprint(self):
+        """Thread-relative points with the outputs they feed, in traversal order."""
+        contrib = defaultdict(list)
+        for o in self.outputs():
+            for (dx, dy, dz) in self.offs:
+                p = (o[0] + dx, o[1] + dy, o[2] + dz)
+                contrib[p].append((o, self.tindex[(dx, dy, dz)],
+                                   spec.mask_index(self.d, dx, dy, dz)))
+        pts = sorted(contrib, key=lambda p: spec.sort_key(self.d, *p))
+        return pts, contrib
+
+    def wname(self, m):
+        return f"w{m}"
+
+    # ----------------------------------------------------------- accumulate
+    def acc_stmt(self, o, t, m, v):
+        """Statement adding term t (value expr v, mask index m) to output o."""
+        i, j, k = o
+        a = f"r_{i}_{j}_{k}"
+        var = self.d.coeff == VARIABLE
+        n = self.d.n
+        if not self.f32:
+            if t == 0:
+                return f"{self.T} {a} = {v} * {self.wname(m)};" if var else f"{self.T} {a} = {v};"
+            return f"{a} = __builtin_fma({v}, {self.wname(m)}, {a});" if var else f"{a} = {a} + {v};"
+        # F32C: groups of g terms summed with one FMA chain, groups combined by TwoSum
+        g = self.d.comp_group
+        gi = t // g
+        q, s, c = f"q{gi}_{i}_{j}_{k}", f"s_{i}_{j}_{k}", f"c_{i}_{j}_{k}"
+        out = []
+        if t % g == 0:
+            out.append(f"float {q} = {v} * {self.wname(m)};" if var else f"float {q} = {v};")
+        else:
+            out.append(f"{q} = __builtin_fmaf({v}, {self.wname(m)}, {q});" if var else f"{q} = {q} + {v};")
+        if t % g == g - 1 or t == n - 1:
+            if gi == 0:
+                out.append(f"float {s} = {q};")
+            elif gi == 1:
+                out.append(f"float {c}; tcsg_two_sum({s}, {q}, {s}, {c});")
+            else:
+                out.append(f"{{ float e_; tcsg_two_sum({s}, {q}, {s}, e_); {c} = {c} + e_; }}")
+        return " ".join(out)
+
+    def final_expr(self, o):
+        i, j, k = o
+        n = self.d.n
+        var = self.d.coeff == VARIABLE
+        if not self.f32:
+            a = f"r_{i}_{j}_{k}"
+            return a if var else f"{a} * {_hexf(1.0 / n, False)}"
+        s, c = f"s_{i}_{j}_{k}", f"c_{i}_{j}_{k}"
+        ngroups = (n + self.d.comp_group - 1) // self.d.comp_group
+        tot = f"({s} + {c})" if ngroups > 1 else s
+        return tot if var else f"{tot} * {_hexf(1.0 / n, True)}"
+
+    # ------------------------------------------------------------ preamble
+    def preamble(self):
+        d = self.d
+        T = self.T
+        w = self.w
+        w("// Generated by threadcoarsenedstencilgenerator_amd.generator -- do not edit.")
+        nbname = "Box" if d.neighbourhood == BOX else "Star"
+        w(f"// {d.dimension}D {nbname} r={d.range} "
+          f"{'Variable' if d.coeff == VARIABLE else 'Jacobi'} "
+          f"{'Row' if d.orientation == ROW else 'Column'} coarsening {d.xc}x{d.yc}x{d.zc} "
+          f"alignment {['None', 'Read', 'Write'][d.alignment]}/{d.cl_size} "
+          f"{'f32c' if self.f32 else 'f64'} {'stream' if d.variant == STREAM else 'plain'} "
+          f"xext={d.xext} xcd_remap={int(d.xcd_remap)} nt={int(d.nt_store)}")
+        w("#include <hip/hip_runtime.h>")
+        w("#include <stdint.h>")
+        w("")
+        w("struct tcsg_kargs {")
+        w(f"  const {T}* in;")
+        w(f"  {T}* out;")
+        w(f"  const {T}* mask;")
+        w("  uint32_t xdim, ydim, zdim, right_offset;")
+        w("  uint32_t in_bytes, out_bytes, pad0, pad1;")
+        w("};")
+        blob = ", ".join(str(v) for v in d.params_blob())
+        w(f'extern "C" __device__ const uint32_t tcsg_params[24] = {{{blob}}};')
+        w("")
+        if self.f32:
+            w("__device__ __forceinline__ void tcsg_two_sum(float a, float b, float& s, float& e) {")
+            w("  const float x = a + b;")
+            w("  const float bp = x - a;")
+            w("  const float ap = x - bp;")
+            w("  e = (a - ap) + (b - bp);")
+            w("  s = x;")
+            w("}")
+        if d.variant == STREAM and d.xext == "dpp":
+            # lanes without a source lane (the wave's ends) receive 0; the kernel
+            # never uses those values
+            for nm, ctrl in (("shr1", DPP_WAVE_SHR1), ("shl1", DPP_WAVE_SHL1)):
+                if self.f32:
+                    w(f"__device__ __forceinline__ float tcsg_{nm}(float v) {{")
+                    w(f"  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp("
+                      f"__builtin_bit_cast(int, v), {ctrl:#x}, 0xf, 0xf, true));")
+                    w("}")
+                else:
+                    w(f"__device__ __forceinline__ double tcsg_{nm}(double v) {{")
+                    w("  const int lo = __double2loint(v), hi = __double2hiint(v);")
+                    w(f"  return __hiloint2double(__builtin_amdgcn_mov_dpp(hi, {ctrl:#x}, 0xf, 0xf, true),")
+                    w(f"                          __builtin_amdgcn_mov_dpp(lo, {ctrl:#x}, 0xf, 0xf, true));")
+                    w("}")
+        if d.variant == STREAM:
+            bits = 32 if self.f32 else 64
+            ity = "int" if self.f32 else "__attribute__((ext_vector_type(2))) int"
+            w(f"__device__ __forceinline__ {T} tcsg_ld(__amdgpu_buffer_rsrc_t r, uint32_t off) {{")
+            w(f"  return __builtin_bit_cast({T}, __builtin_amdgcn_raw_buffer_load_b{bits}(r, off, 0, 0));")
+            w("}")
+            aux = 2 if d.nt_store else 0
+            w(f"__device__ __forceinline__ void tcsg_st(__amdgpu_buffer_rsrc_t r, uint32_t off, {T} v) {{")
+            w(f"  __builtin_amdgcn_raw_buffer_store_b{bits}(__builtin_bit_cast({ity}, v), r, off, 0, {aux});")
+            w("}")
+        w("")
+
+    # ------------------------------------------------------------ body parts
+    def block_coords(self):
+        w = self.w
+        if self.d.xcd_remap:
+            w("  // XCD-aware remap: hardware hands consecutive workgroups to the 8 XCDs")
+            w("  // round-robin; give each XCD a contiguous range of tiles instead.")
+            w("  const uint32_t nbx = gridDim.x, nby = gridDim.y, nbz = gridDim.z;")
+            w("  const uint32_t nblk = nbx * nby * nbz;")
+            w("  uint32_t lin = blockIdx.x + nbx * (blockIdx.y + nby * blockIdx.z);")
+            w("  {")
+            w("    const uint32_t q = nblk >> 3, rm = nblk & 7u, xcd = lin & 7u, k = lin >> 3;")
+            w("    lin = xcd < rm ? xcd * (q + 1u) + k : rm * (q + 1u) + (xcd - rm) * q + k;")
+            w("  }")
+            w("  const uint32_t bxi = lin % nbx, byi = (lin / nbx) % nby, bzi = lin / (nbx * nby);")
+        else:
+            w("  const uint32_t bxi = blockIdx.x, byi = blockIdx.y, bzi = blockIdx.z;")
+        w("  const uint32_t gx = bxi * blockDim.x + threadIdx.x;")
+        w("  const uint32_t gy = byi * blockDim.y + threadIdx.y;")
+        w("  const uint32_t gz = bzi * blockDim.z + threadIdx.z;")
+
+    def thread_counts(self):
+        d = self.d
+        R, LO = self.R, d.left_offset
+        w = self.w
+        w("  const uint32_t xdim = a.xdim, ydim = a.ydim;")
+        w(f"  const uint32_t ex = xdim - a.right_offset;  // end of the x extent in use")
+        w(f"  const uint32_t thx = ex >= {2 * R + LO} ? (ex - {2 * R + LO}) / {self.XC}u : 0u;")
+        w(f"  const uint32_t thy = ydim >= {2 * R} ? (ydim - {2 * R}) / {self.YC}u : 0u;")
+        if self.three:
+            w("  const uint32_t zdim = a.zdim;")
+            w(f"  const uint32_t thz = zdim >= {2 * R} ? (zdim - {2 * R}) / {self.ZC}u : 0u;")
+        else:
+            w("  const uint32_t thz = 1u;")
+
+    # ---------------------------------------------------------------- PLAIN
+    def body_plain(self):
+        d = self.d
+        T = self.T
+        w = self.w
+        R, LO = self.R, d.left_offset
+        self.thread_counts()
+        self.block_coords()
+        w("  if (!(gx < thx && gy < thy && gz < thz)) return;")
+        w(f"  const size_t x = (size_t)gx * {self.XC} + {R + LO};")
+        w(f"  const size_t y = (size_t)gy * {self.YC} + {R};")
+        w(f"  const size_t z = (size_t)gz * {self.ZC} + {R};" if self.three else "  const size_t z = 0;")
+        w("  const size_t sx = xdim;")
+        w("  const size_t sxy = (size_t)xdim * ydim;")
+        w(f"  const {T}* __restrict__ src = a.in + z * sxy + y * sx + x;")
+        w(f"  {T}* __restrict__ dst = a.out + z * sxy + y * sx + x;")
+        self.weights()
+        pts, contrib = self.footprint()
+        last = {}
+        for idx, p in enumerate(pts):
+            for (o, t, m) in contrib[p]:
+                last[o] = idx
+        rows = set()
+        for idx, p in enumerate(pts):
+            px, py, pz = p
+            rk = (py, pz)
+            if rk not in rows:
+                rows.add(rk)
+                w(f"  const {T}* __restrict__ q_{_n(py)}_{_n(pz)} = src + (ptrdiff_t)({py}) * (ptrdiff_t)sx"
+                  f" + (ptrdiff_t)({pz}) * (ptrdiff_t)sxy;")
+            v = f"v_{_n(px)}_{_n(py)}_{_n(pz)}"
+            w(f"  const {T} {v} = q_{_n(py)}_{_n(pz)}[{px}];")
+            for (o, t, m) in sorted(contrib[p], key=lambda c: c[0]):
+                w("  " + self.acc_stmt(o, t, m, v))
+            for o in self.outputs():
+                if last[o] == idx:
+                    i, j, k = o
+                    w(f"  dst[(ptrdiff_t)({k}) * (ptrdiff_t)sxy + (ptrdiff_t)({j}) * (ptrdiff_t)sx + {i}]"
+                      f" = {self.final_expr(o)};")
+
+    def weights(self):
+        if self.d.coeff != VARIABLE:
+            return
+        T = self.T
+        self.w(f"  const {T}* __restrict__ mk = a.mask;")
+        for m in range(self.d.n):
+            self.w(f"  const {T} w{m} = mk[{m}];")
+
+    # --------------------------------------------------------------- STREAM
+    def halo_lanes(self) -> int:
+        """H = ceil(r / xc): lanes a thread reaches for its x-neighbours."""
+        return (self.R + self.XC - 1) // self.XC
+
+    def body_stream(self):
+        d = self.d
+        T = self.T
+        w = self.w
+        R, LO = self.R, d.left_offset
+        XC = self.XC
+        es = 4 if self.f32 else 8
+        dpp = d.xext == "dpp"
+        H = self.halo_lanes()
+        self.thread_counts()
+        self.block_coords()
+        w("  // inrow: the thread's row/plane is inside the interior; such lanes load (and")
+        w("  // can feed x-neighbours to other lanes) even past the interior's x end, where")
+        w("  // they read the halo.  valid: the thread owns interior points and stores them.")
+        w("  const bool inrow = gy < thy && gz < thz;")
+        w("  const bool valid = inrow && gx < thx;")
+        w(f"  const uint32_t x0 = gx * {XC}u + {R + LO}u;")
+        w(f"  const uint32_t y0 = gy * {self.YC}u + {R}u;")
+        w(f"  const uint32_t z0 = gz * {self.ZC}u + {R}u;" if self.three else "  const uint32_t z0 = 0u;")
+        w(f"  const uint32_t sx = xdim * {es}u;            // row pitch in bytes")
+        w(f"  const uint32_t sxy = xdim * ydim * {es}u;     // plane pitch in bytes")
+        w(f"  const uint32_t real = (z0 * ydim + y0) * xdim * {es}u + x0 * {es}u;")
+        w(f"  // offsets past the buffer: loads read 0, stores are dropped by the range check")
+        w(f"  const uint32_t base = inrow ? real : {OOB_BASE:#x}u;")
+        w(f"  const uint32_t sbase = valid ? real : {OOB_BASE:#x}u;")
+        w("  const __amdgpu_buffer_rsrc_t rin = __builtin_amdgcn_make_buffer_rsrc((void*)a.in, (short)0, (int)a.in_bytes, 0x00020000);")
+        w("  const __amdgpu_buffer_rsrc_t rout = __builtin_amdgcn_make_buffer_rsrc((void*)a.out, (short)0, (int)a.out_bytes, 0x00020000);")
+        self.weights()
+        if dpp:
+            w("  // Lane segments: runs of lanes in one block row and one wave.  tL/tR are the")
+            w("  // lane's distances to its segment's ends; the runtime only launches block")
+            w(f"  // shapes whose segments are at least H = {H} lanes long.")
+            w("  const uint32_t lid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);")
+            w("  const uint32_t lane = lid & 63u;")
+            w("  const uint32_t tx = threadIdx.x;")
+            w("  const uint32_t tL = min(tx, lane);")
+            w("  const uint32_t tR = min(blockDim.x - 1u - tx, 63u - lane);")
+            w(f"  const uint32_t eoffL = tL < {H}u ? base - {H * XC * es}u : {OOB_BASE:#x}u;")
+            w(f"  const uint32_t eoffR = tR < {H}u ? base + {H * XC * es}u : {OOB_BASE:#x}u;")
+        pts, contrib = self.footprint()
+        last = {}
+        for idx, p in enumerate(pts):
+            for (o, t, m) in contrib[p]:
+                last[o] = idx
+        # acquisition groups: Row -> rows (z, y); Column -> planes (z)
+        if d.orientation == ROW:
+            gkey = lambda p: (p[2], p[1])
+        else:
+            gkey = lambda p: (p[2],)
+        groups = []
+        for idx, p in enumerate(pts):
+            if not groups or groups[-1][0] != gkey(p):
+                groups.append((gkey(p), []))
+            groups[-1][1].append((idx, p))
+        declared = set()
+
+        def rowoff(py, pz):
+            return f"(uint32_t)({py}) * sx + (uint32_t)({pz}) * sxy"
+
+        def rd(py, pz, prefix="d", b="base"):
+            k = f"{prefix}_{_n(py)}_{_n(pz)}"
+            if k not in declared:
+                declared.add(k)
+                w(f"  const uint32_t {k} = {b} + {rowoff(py, pz)};")
+            return k
+
+        def vname(px, py, pz):
+            return f"v_{_n(px)}_{_n(py)}_{_n(pz)}"
+
+        def hop(px):
+            """(side, h, ip): lane hop and source column of an x-neighbour."""
+            if px < 0:
+                h = (-px + XC - 1) // XC
+                return "L", h, h * XC + px
+            h = px // XC
+            return "R", h, px - h * XC
+
+        for _, members in groups:
+            gpts = [p for _, p in members]
+            need_own = set()
+            for (px, py, pz) in gpts:
+                if 0 <= px < XC or not dpp:
+                    need_own.add((px, py, pz))
+                else:
+                    need_own.add((hop(px)[2], py, pz))
+            for (px, py, pz) in sorted(need_own, key=lambda p: spec.sort_key(d, *p)):
+                nm = vname(px, py, pz)
+                if nm in declared:
+                    continue
+                declared.add(nm)
+                dk = rd(py, pz)
+                w(f"  const {T} {nm} = tcsg_ld(rin, {dk} + {px * es}u);" if px >= 0 else
+                  f"  const {T} {nm} = tcsg_ld(rin, {dk} - {-px * es}u);")
+            if dpp:
+                for (px, py, pz) in gpts:
+                    nm = vname(px, py, pz)
+                    if 0 <= px < XC or nm in declared:
+                        continue
+                    side, h, ip = hop(px)
+                    main_fn, e_fn = ("tcsg_shr1", "tcsg_shl1") if side == "L" else ("tcsg_shl1", "tcsg_shr1")
+                    # main chain: value of lane t-h (left) / t+h (right)
+                    prev = vname(ip, py, pz)
+                    for hh in range(1, h + 1):
+                        sn = f"s{side}{hh}_{ip}_{_n(py)}_{_n(pz)}"
+                        if sn not in declared:
+                            declared.add(sn)
+                            w(f"  const {T} {sn} = {main_fn}({prev});")
+                        prev = sn
+                    # halo chain: extra columns loaded by the H edge lanes of each segment
+                    en = f"e{side}0_{ip}_{_n(py)}_{_n(pz)}"
+                    if en not in declared:
+                        declared.add(en)
+                        ek = f"eoff{side}"
+                        w(f"  const {T} {en} = tcsg_ld(rin, {ek} + {rowoff(py, pz)} + {ip * es}u);")
+                    eprev = en
+                    for j in range(1, H - h + 1):
+                        gn = f"e{side}{j}_{ip}_{_n(py)}_{_n(pz)}"
+                        if gn not in declared:
+                            declared.add(gn)
+                            w(f"  const {T} {gn} = {e_fn}({eprev});")
+                        eprev = gn
+                    w(f"  const {T} {nm} = t{side} >= {h}u ? {prev} : {eprev};")
+                    declared.add(nm)
+            for idx, p in members:
+                v = vname(*p)
+                for (o, t, m) in sorted(contrib[p], key=lambda c: c[0]):
+                    w("  " + self.acc_stmt(o, t, m, v))
+                for o in self.outputs():
+                    if last[o] == idx:
+                        i, j, k = o
+                        sk = rd(j, k, "sd", "sbase")
+                        w(f"  tcsg_st(rout, {sk} + {i * es}u, {self.final_expr(o)});")
+
+    # ------------------------------------------------------------------ all
+    def source(self) -> str:
+        d = self.d
+        self.preamble()
+        self.w(f'extern "C" __global__ void __launch_bounds__({d.max_threads}) tcsg_update(const tcsg_kargs a)')
+        self.w("{")
+        if d.variant == PLAIN:
+            self.body_plain()
+        else:
+            self.body_stream()
+        self.w("}")
+        return "\n".join(self.L) + "\n"
+
+
+def emit_source(desc: StencilDesc) -> str:
+    """HIP source text of the kernel for ``desc``."""
+    return _Emitter(desc).source()

@@ -1,0 +1,102 @@
+"""The kernels built ahead of time (in-tree code objects under kernels/).
+
+Generated kernels are compile-time specialisations; the parity suite and the
+benchmark workloads use the sets below, which ``__graft_entry__.build()``
+compiles with hipcc for gfx950.
+"""
+from __future__ import annotations
+
+import itertools
+
+from .spec import (ALIGN_NONE, ALIGN_READ, ALIGN_WRITE, BOX, COLUMN, F32C, F64, JACOBI, PLAIN, ROW,
+                   STAR, STREAM, VARIABLE, StencilDesc)
+
+# code shapes every parity case is built in
+SHAPES = {
+    "plain": dict(variant=PLAIN),
+    "stream": dict(variant=STREAM, xext="load", xcd_remap=True, nt_store=True),
+    "dpp": dict(variant=STREAM, xext="dpp", xcd_remap=True, nt_store=True),
+}
+
+
+def parity_cases():
+    """(desc, grid size incl. halo) pairs covering the option matrix on small grids."""
+    out = []
+    for dim, nb, orient, coeff in itertools.product((2, 3), (BOX, STAR), (ROW, COLUMN), (JACOBI, VARIABLE)):
+        for shape in SHAPES.values():
+            # two parameter points per neighbourhood/orientation/coefficient cell
+            pts = [
+                dict(range=1, xc=1, yc=1, zc=1, alignment=ALIGN_NONE, cl_size=0),
+                dict(range=2, xc=2, yc=3, zc=2, alignment=ALIGN_WRITE, cl_size=4),
+            ]
+            if nb == STAR:
+                pts.append(dict(range=3, xc=1, yc=2, zc=3, alignment=ALIGN_READ, cl_size=8))
+            for p in pts:
+                if dim == 2:
+                    p = dict(p, zc=1)
+                d = StencilDesc(dimension=dim, neighbourhood=nb, orientation=orient, coeff=coeff, **p, **shape)
+                size = (37, 29) if dim == 2 else (23, 19, 17)
+                out.append((d, size))
+    return out
+
+
+def f32c_cases():
+    out = []
+    for dim, nb, coeff, orient in ((3, STAR, VARIABLE, ROW), (2, BOX, VARIABLE, COLUMN), (3, BOX, JACOBI, ROW),
+                                   (2, STAR, JACOBI, ROW)):
+        for shape in ("plain", "dpp"):
+            d = StencilDesc(dimension=dim, neighbourhood=nb, orientation=orient, coeff=coeff, range=2, xc=2,
+                            yc=2, zc=2 if dim == 3 else 1, alignment=ALIGN_WRITE, cl_size=16, dtype=F32C,
+                            **SHAPES[shape])
+            out.append((d, (45, 33) if dim == 2 else (27, 21, 19)))
+    return out
+
+
+# ---- benchmark workloads ----
+def cfg1_desc(shape="plain"):
+    """2D Box Jacobi r=1, coarsening 1x1, Row, alignment None."""
+    return StencilDesc(dimension=2, neighbourhood=BOX, orientation=ROW, coeff=JACOBI, range=1, xc=1, yc=1,
+                       alignment=ALIGN_NONE, cl_size=0, **SHAPES[shape])
+
+
+def cfg2_desc(xc=1, yc=1, zc=1, shape="plain", dtype=F64, max_threads=1024, orientation=ROW):
+    """3D Star Variable r=4, alignment Write at 16 points."""
+    return StencilDesc(dimension=3, neighbourhood=STAR, orientation=orientation, coeff=VARIABLE, range=4,
+                       xc=xc, yc=yc, zc=zc, alignment=ALIGN_WRITE, cl_size=16, dtype=dtype,
+                       max_threads=max_threads, **SHAPES[shape])
+
+
+def cfg2_sweep():
+    """The coarsening sweep x{1,2} y{1,2,4} z{1,2,4,8} of the 512^3 workload (reference shape)."""
+    return [cfg2_desc(x, y, z) for x in (1, 2) for y in (1, 2, 4) for z in (1, 2, 4, 8)]
+
+
+def cfg2_tuned():
+    """MI355X code shapes for the 512^3 workload."""
+    out = []
+    for (x, y, z) in ((1, 4, 16), (1, 2, 16), (2, 2, 16), (1, 4, 32), (2, 4, 16), (1, 8, 16), (4, 2, 16),
+                      (2, 2, 32), (1, 2, 32)):
+        for mt in (256, 512):
+            out.append(cfg2_desc(x, y, z, shape="dpp", max_threads=mt))
+    return out
+
+
+def cfg3_desc(xc=1, yc=1, orientation=ROW, dtype=F64, shape="plain", max_threads=1024):
+    """2D Box Variable r=8, alignment Read at 16 points."""
+    return StencilDesc(dimension=2, neighbourhood=BOX, orientation=orientation, coeff=VARIABLE, range=8, xc=xc,
+                       yc=yc, alignment=ALIGN_READ, cl_size=16, dtype=dtype, max_threads=max_threads,
+                       **SHAPES[shape])
+
+
+def cfg4_desc(dtype=F64, shape="dpp"):
+    """3D Box Jacobi r=2, coarsening 2x2x4."""
+    return StencilDesc(dimension=3, neighbourhood=BOX, orientation=ROW, coeff=JACOBI, range=2, xc=2, yc=2,
+                       zc=4, alignment=ALIGN_NONE, cl_size=0, dtype=dtype, **SHAPES[shape])
+
+
+def default_build_set():
+    descs = [d for d, _ in parity_cases()] + [d for d, _ in f32c_cases()]
+    descs += [cfg1_desc("plain"), cfg1_desc("stream"), cfg1_desc("dpp")]
+    descs += cfg2_sweep() + cfg2_tuned()
+    descs += [cfg4_desc(F64, "plain"), cfg4_desc(F64, "dpp")]
+    return list(dict.fromkeys(descs))

@@ -1,0 +1,259 @@
+"""Stencil descriptor, neighbourhood, mask layout and geometry.
+
+These are the parameters of the reference's generator CLIs
+(hip/generate2DStencil_hip.py:28-60, hip/generate3DStencil_hip.py:29-63) and
+harness (hip/benchmark.cpp:62-81,118-218).  The generator uses them to emit
+kernels; the C runtime restates the geometry part in C for the launch path.
+"""
+from __future__ import annotations
+
+import dataclasses
+import hashlib
+
+BOX, STAR = 0, 1
+ROW, COLUMN = 0, 1
+JACOBI, VARIABLE = 0, 1
+ALIGN_NONE, ALIGN_READ, ALIGN_WRITE = 0, 1, 2
+F64, F32C = 0, 1
+PLAIN, STREAM = 0, 1
+
+NB_NAMES = {"Box": BOX, "Star": STAR}
+ORIENT_NAMES = {"Row": ROW, "Column": COLUMN}
+COEFF_NAMES = {"Jacobi": JACOBI, "Variable": VARIABLE}
+ALIGN_NAMES = {"None": ALIGN_NONE, "Read": ALIGN_READ, "Write": ALIGN_WRITE}
+DTYPE_NAMES = {"f64": F64, "f32c": F32C}
+VARIANT_NAMES = {"plain": PLAIN, "stream": STREAM}
+
+
+def _inv(d):
+    return {v: k for k, v in d.items()}
+
+
+@dataclasses.dataclass(frozen=True)
+class StencilDesc:
+    """One generated kernel's compile-time parameters.
+
+    dimension/neighbourhood/orientation/coeff/range/coarsening/alignment/
+    cl_size mirror the reference generator argv.  dtype, variant and the
+    tuning knobs (max_threads, xext, xcd_remap, nt_store, comp_group) have no
+    reference counterpart: they select the MI355X code shape.
+    """
+
+    dimension: int = 3
+    neighbourhood: int = STAR
+    orientation: int = ROW
+    coeff: int = VARIABLE
+    range: int = 1
+    xc: int = 1
+    yc: int = 1
+    zc: int = 1
+    alignment: int = ALIGN_NONE
+    cl_size: int = 0
+    dtype: int = F64
+    variant: int = PLAIN
+    max_threads: int = 1024
+    xext: str = "load"       # "load" | "dpp": how x-neighbours outside the thread tile are obtained
+    xcd_remap: bool = False  # remap block ids so neighbouring tiles share an XCD's L2
+    nt_store: bool = False   # nontemporal stores of the outputs
+    comp_group: int = 5      # F32C: terms summed per group before the TwoSum combine
+
+    def __post_init__(self):
+        if self.dimension not in (2, 3):
+            raise ValueError("dimension must be 2 or 3")
+        if self.neighbourhood not in (BOX, STAR):
+            raise ValueError("unknown stencil neighbourhood")
+        if self.orientation not in (ROW, COLUMN):
+            raise ValueError("unknown access orientation")
+        if self.coeff not in (JACOBI, VARIABLE):
+            raise ValueError("unknown coefficient type")
+        if self.range < 1:
+            raise ValueError("range has to be at least 1")
+        if self.xc < 1 or self.yc < 1 or self.zc < 1:
+            raise ValueError("coarsening factors have to be at least 1")
+        if self.dimension == 2 and self.zc != 1:
+            raise ValueError("2D stencils have no z coarsening")
+        if self.alignment not in (ALIGN_NONE, ALIGN_READ, ALIGN_WRITE):
+            raise ValueError("unknown alignment")
+        if self.alignment != ALIGN_NONE and self.cl_size <= 0:
+            raise ValueError("size of alignment has to be > 0")
+        if self.xext not in ("load", "dpp"):
+            raise ValueError("xext must be 'load' or 'dpp'")
+        if self.max_threads % 64 or not 64 <= self.max_threads <= 1024:
+            raise ValueError("max_threads must be a multiple of 64 in [64, 1024]")
+        if self.comp_group < 1:
+            raise ValueError("comp_group must be >= 1")
+
+    # ---- derived quantities ----
+    @property
+    def n(self) -> int:
+        """Stencil size, hip/benchmark.cpp:62-73."""
+        r = self.range
+        if self.neighbourhood == BOX:
+            return (2 * r + 1) ** self.dimension
+        return 2 * self.dimension * r + 1
+
+    @property
+    def flops_per_update(self) -> int:
+        """hip/benchmark.cpp:75-80: Jacobi n, Variable 2n-1."""
+        return self.n if self.coeff == JACOBI else 2 * self.n - 1
+
+    @property
+    def left_offset(self) -> int:
+        """Write alignment pads rows on the left so the first interior point
+        starts a line: hip/benchmark.h:110-116, hip/generate2DStencil_hip.py:83-85."""
+        if self.alignment == ALIGN_WRITE and self.range % self.cl_size:
+            return self.cl_size - self.range % self.cl_size
+        return 0
+
+    @property
+    def coarsening(self):
+        return (self.xc, self.yc, self.zc)
+
+    def key(self) -> str:
+        """Stable short name used for code-object files."""
+        d = dataclasses.asdict(self)
+        h = hashlib.sha1(repr(sorted(d.items())).encode()).hexdigest()[:10]
+        nb = "box" if self.neighbourhood == BOX else "star"
+        return (f"{self.dimension}d_{nb}_r{self.range}_{_inv(COEFF_NAMES)[self.coeff].lower()}"
+                f"_{_inv(ORIENT_NAMES)[self.orientation].lower()}_c{self.xc}x{self.yc}x{self.zc}"
+                f"_{_inv(ALIGN_NAMES)[self.alignment].lower()}{self.cl_size}"
+                f"_{_inv(DTYPE_NAMES)[self.dtype]}_{_inv(VARIANT_NAMES)[self.variant]}_{h}")
+
+    def params_blob(self):
+        """Integers baked into the code object and checked by the runtime
+        before launch (the reference's compiled-in sanity check,
+        hip/generateLaunchUpdateGrid.py:106-119)."""
+        return [0x54435347, 1, self.dimension, self.neighbourhood, self.orientation,
+                self.coeff, self.range, self.xc, self.yc, self.zc, self.alignment,
+                self.cl_size, self.dtype, self.variant, self.n, self.max_threads,
+                int(self.xext == "dpp"), int(self.xcd_remap), int(self.nt_store),
+                self.comp_group, 0, 0, 0, 0]
+
+    def replace(self, **kw) -> "StencilDesc":
+        return dataclasses.replace(self, **kw)
+
+
+def mask_index(desc: StencilDesc, dx: int, dy: int, dz: int = 0):
+    """Position of offset (dx,dy,dz) in the mask: rank in (dz,dy,dx)
+    lexicographic order over the neighbourhood.  Box: row-major
+    (hip/generate2DStencil_hip.py:124-128, hip/generate3DStencil_hip.py:137-142);
+    Star: the arm blocks of calcMaskAddress (2D :129-144, 3D :143-169).
+    Returns None for offsets outside the neighbourhood."""
+    r = desc.range
+    if desc.dimension == 2:
+        if dz:
+            return None
+    if max(abs(dx), abs(dy), abs(dz)) > r:
+        return None
+    w = 2 * r + 1
+    if desc.neighbourhood == BOX:
+        if desc.dimension == 2:
+            return (dx + r) + (dy + r) * w
+        return (dx + r) + (dy + r) * w + (dz + r) * w * w
+    if (dx != 0) + (dy != 0) + (dz != 0) > 1:
+        return None
+    if desc.dimension == 2:
+        if dy < 0:
+            return dy + r
+        if dy == 0:
+            return r + dx + r
+        return dy + r + 2 * r
+    if dz < 0:
+        return dz + r
+    if dz == 0 and dy < 0:
+        return dy + 2 * r
+    if dz == 0 and dy == 0:
+        return dx + 3 * r
+    if dz == 0:
+        return dy + 4 * r
+    return dz + 5 * r
+
+
+def in_neighbourhood(desc: StencilDesc, dx, dy, dz=0) -> bool:
+    return mask_index(desc, dx, dy, dz) is not None
+
+
+def sort_key(desc: StencilDesc, x, y, z=0):
+    """Traversal order key.  Row visits (z, y, x) lexicographically, Column
+    (z, x, y): the order of the reference's loop nests
+    (hip/generate2DStencil_hip.py:245-298, hip/generate3DStencil_hip.py:266-361).
+    Translating by an output position keeps lexicographic order, so this is
+    also each output's summation order over its offsets."""
+    return (z, y, x) if desc.orientation == ROW else (z, x, y)
+
+
+def offsets(desc: StencilDesc):
+    """Neighbourhood offsets (dx,dy,dz) in per-output summation order."""
+    r = desc.range
+    zr = r if desc.dimension == 3 else 0
+    pts = [(dx, dy, dz) for dz in range(-zr, zr + 1) for dy in range(-r, r + 1)
+           for dx in range(-r, r + 1) if in_neighbourhood(desc, dx, dy, dz)]
+    pts.sort(key=lambda p: sort_key(desc, *p))
+    assert len(pts) == desc.n
+    return pts
+
+
+def jacobi_c(desc: StencilDesc) -> float:
+    """fl(1/n), the literal the reference prints via Python repr
+    (hip/generate2DStencil_hip.py:300-307, hip/generate3DStencil_hip.py:363-370)."""
+    return 1.0 / desc.n
+
+
+@dataclasses.dataclass
+class Geometry:
+    """Padded grid geometry (hip/benchmark.cpp:140-177, hip/benchmark.h:100-123).
+
+    size      user grid size including the halo, per axis
+    xdim..    padded allocation extents (x includes left/right padding)
+    lo, ro    left / right padding in points
+    elements  interior (updated) points per axis
+    threads   threads per axis = elements / coarsening
+    """
+
+    dimension: int
+    range: int
+    xdim: int
+    ydim: int
+    zdim: int
+    lo: int
+    ro: int
+    elements: tuple
+    threads: tuple
+
+    @property
+    def n_points(self) -> int:
+        return self.xdim * self.ydim * self.zdim
+
+    @property
+    def interior_points(self) -> int:
+        e = self.elements
+        return e[0] * e[1] * e[2]
+
+
+def geometry(desc: StencilDesc, size) -> Geometry:
+    r = desc.range
+    size = list(size) + [1] * (3 - len(size))
+    c = desc.coarsening
+    t = [1, 1, 1]
+    e = [1, 1, 1]
+    for a in range(desc.dimension):
+        if size[a] <= 2 * r:
+            raise ValueError(f"grid axis {a} ({size[a]}) too small for range {r}")
+        t[a] = size[a] - (size[a] - 2 * r) % c[a]
+        e[a] = t[a] - 2 * r
+    lo = ro = 0
+    xmem = t[0]
+    cl = desc.cl_size
+    if desc.alignment == ALIGN_READ:
+        if xmem % cl:
+            ro = cl - xmem % cl
+            xmem += ro
+    elif desc.alignment == ALIGN_WRITE:
+        lo = desc.left_offset
+        xmem += lo
+        if xmem % cl:
+            ro = cl - xmem % cl
+        xmem += ro
+    thr = tuple(e[a] // c[a] if a < desc.dimension else 1 for a in range(3))
+    return Geometry(desc.dimension, r, xmem, t[1], t[2] if desc.dimension == 3 else 1,
+                    lo, ro, tuple(e), thr)

@@ -91,6 +91,12 @@ typedef struct tcsg_kernel tcsg_kernel;
  * (hip/benchmark.cpp:133-182). */
 int tcsg_geometry_compute(const tcsg_desc *desc, tcsg_dim3 size, tcsg_geometry *out);
 
+/* TCSG_OK if *g is the geometry this descriptor produces for some user grid
+ * size and, for TCSG_STREAM kernels, fits their 32-bit buffer addressing;
+ * TCSG_ERR_ARG / TCSG_ERR_TOO_LARGE otherwise.  Every launch entry point
+ * applies this check. */
+int tcsg_geometry_check(const tcsg_desc *desc, const tcsg_geometry *g);
+
 /* Load a generated code object on the current device and check its baked-in
  * parameters against *expect (NULL: accept).  Replaces the compile-time
  * binding of kernels.cu into launchUpdateGrid.cu

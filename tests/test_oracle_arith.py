@@ -1,0 +1,299 @@
+"""Pins the oracle's arithmetic and geometry with checks that do not restate
+its code: delta masks (exact shifts), closed-form answers for the
+reference's init (hip/generate{2,3}DStencil_hip.py init kernels), a per-point
+brute force with exactly rounded FMA (fractions), isotropic Variable vs
+Jacobi (the hip/benchmark.cpp:123-125 remark), linearity, the geometry's
+worked examples and invariants, and TwoSum's exactness for the FP32
+compensated variant."""
+import itertools
+import math
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import ALIGN_NONE, ALIGN_READ, ALIGN_WRITE, BOX, COLUMN, JACOBI, ROW, STAR, VARIABLE
+
+
+class D:
+    """Minimal descriptor for the oracle wrapper."""
+
+    def __init__(self, dim, nb, orient, coeff, r, comp_group=5):
+        self.dimension, self.neighbourhood, self.orientation = dim, nb, orient
+        self.coeff, self.range, self.comp_group = coeff, r, comp_group
+
+
+def _grid(dim, r, align=ALIGN_NONE, cl=0, size=None, C=(1, 1, 1)):
+    size = size or ((13, 11) if dim == 2 else (9, 8, 7))
+    return oracle.Geometry(dim, r, align, cl, size, C)
+
+
+def _interior(g):
+    zr = g.r if g.dim == 3 else 0
+    for k in range(zr, g.zdim - zr):
+        for j in range(g.r, g.ydim - g.r):
+            for i in range(g.lo + g.r, g.xdim - g.ro - g.r):
+                yield i, j, k
+
+
+def _idx(g, i, j, k):
+    return i + j * g.xdim + k * g.xdim * g.ydim
+
+
+@pytest.mark.parametrize("dim,nb,orient", list(itertools.product((2, 3), (BOX, STAR), (ROW, COLUMN))))
+def test_delta_mask_is_exact_shift(dim, nb, orient):
+    """A mask that is 1 at offset o and 0 elsewhere gives out[p] = in[p+o]:
+    pins the mask layout and the correlation direction."""
+    r = 2
+    g = _grid(dim, r, ALIGN_WRITE, 4)
+    a = np.zeros(g.n_points)
+    oracle.init_seeded(g, 1, 11, a)
+    off, midx = oracle.offsets(dim, nb, orient, r)
+    for o, m in zip(off, midx):
+        mask = np.zeros(len(midx))
+        mask[m] = 1.0
+        out = np.full(g.n_points, np.nan)
+        oracle.step_f64(D(dim, nb, orient, VARIABLE, r), g, mask, a, out)
+        dx, dy, dz = (int(v) for v in o)
+        for (i, j, k) in _interior(g):
+            assert out[_idx(g, i, j, k)] == a[_idx(g, i + dx, j + dy, k + dz)]
+
+
+@pytest.mark.parametrize("dim,nb", list(itertools.product((2, 3), (BOX, STAR))))
+def test_reference_init_one_jacobi_step_closed_form(dim, nb):
+    """Reference init + one Jacobi step: every output is (sum of its halo
+    neighbours' init values) * fl(1/n); the sum is exact (multiples of 1/4)."""
+    r = 1 if dim == 3 else 2
+    g = _grid(dim, r, ALIGN_WRITE, 8 if dim == 2 else 4)
+    a = np.zeros(g.n_points)
+    oracle.init_reference(g, a)
+    out = a.copy()
+    oracle.step_f64(D(dim, nb, ROW, JACOBI, r), g, None, a, out)
+    n = oracle.stencil_size(dim, nb, r)
+    c = 1.0 / n
+    off, _ = oracle.offsets(dim, nb, ROW, r)
+    for (i, j, k) in _interior(g):
+        s = Fraction(0)
+        for dx, dy, dz in off:
+            ii, jj, kk = i + int(dx), j + int(dy), k + int(dz)
+            halo = (ii - g.lo < r) or jj < r or (dim == 3 and kk < r)
+            if halo:
+                idx = _idx(g, ii, jj, kk)
+                s += Fraction(1) if dim == 2 else (Fraction(3, 4) if idx % 2 == 0 else Fraction(1, 4))
+        assert out[_idx(g, i, j, k)] == float(s) * c
+
+
+def _fma(a, b, c):
+    return float(Fraction(a) * Fraction(b) + Fraction(c))
+
+
+@pytest.mark.parametrize("dim,nb,orient,coeff",
+                         list(itertools.product((2, 3), (BOX, STAR), (ROW, COLUMN), (JACOBI, VARIABLE))))
+def test_brute_force_tiny(dim, nb, orient, coeff):
+    """Per-point loop with an exactly rounded FMA, on a padded tiny grid."""
+    r = 2 if dim == 2 else 1
+    g = _grid(dim, r, ALIGN_READ, 4, size=(11, 9) if dim == 2 else (7, 6, 6))
+    a = np.zeros(g.n_points)
+    oracle.init_seeded(g, 1, 5, a)
+    n = oracle.stencil_size(dim, nb, r)
+    mask = oracle.mask_seeded(9, n)
+    out = np.zeros(g.n_points)
+    oracle.step_f64(D(dim, nb, orient, coeff, r), g, mask, a, out)
+    off, midx = oracle.offsets(dim, nb, orient, r)
+    c = 1.0 / n
+    for (i, j, k) in _interior(g):
+        vals = [a[_idx(g, i + int(o[0]), j + int(o[1]), k + int(o[2]))] for o in off]
+        if coeff == VARIABLE:
+            acc = vals[0] * mask[midx[0]]
+            for v, m in zip(vals[1:], midx[1:]):
+                acc = _fma(v, mask[m], acc)
+        else:
+            acc = vals[0]
+            for v in vals[1:]:
+                acc = acc + v
+            acc = acc * c
+        assert out[_idx(g, i, j, k)] == acc
+
+
+def test_isotropic_variable_close_to_jacobi():
+    """hip/benchmark.cpp:123-125: the 1/n test mask gives the Jacobi result
+    up to rounding."""
+    for dim, nb in itertools.product((2, 3), (BOX, STAR)):
+        r = 2
+        g = _grid(dim, r)
+        a = np.zeros(g.n_points)
+        oracle.init_seeded(g, 0, 3, a)
+        n = oracle.stencil_size(dim, nb, r)
+        o1, o2 = np.zeros_like(a), np.zeros_like(a)
+        oracle.step_f64(D(dim, nb, ROW, JACOBI, r), g, None, a, o1)
+        oracle.step_f64(D(dim, nb, ROW, VARIABLE, r), g, np.full(n, 1.0 / n), a, o2)
+        assert np.allclose(o1, o2, rtol=8 * n * 2.0 ** -53, atol=0)
+
+
+def test_linearity():
+    g = _grid(3, 2, ALIGN_WRITE, 16, size=(12, 11, 10))
+    u = np.zeros(g.n_points)
+    v = np.zeros(g.n_points)
+    oracle.init_seeded(g, 1, 1, u)
+    oracle.init_seeded(g, 1, 2, v)
+    d = D(3, STAR, ROW, VARIABLE, 2)
+    mask = oracle.mask_seeded(4, 13)
+    ou, ov, ow = (np.zeros(g.n_points) for _ in range(3))
+    oracle.step_f64(d, g, mask, u, ou)
+    oracle.step_f64(d, g, mask, v, ov)
+    oracle.step_f64(d, g, mask, 0.5 * u - 2.0 * v, ow)
+    assert np.allclose(ow, 0.5 * ou - 2.0 * ov, atol=1e-14)
+
+
+def test_constant_field_preserved_by_jacobi():
+    g = _grid(2, 3)
+    a = np.full(g.n_points, 0.375)
+    out = np.zeros_like(a)
+    oracle.step_f64(D(2, BOX, COLUMN, JACOBI, 3), g, None, a, out)
+    inner = [out[_idx(g, i, j, k)] for (i, j, k) in _interior(g)]
+    assert max(abs(x - 0.375) for x in inner) <= 2 * 2.0 ** -53
+
+
+def test_geometry_worked_examples():
+    # 512^3 interior, r=4, Write at 16: 520 -> 12 left + 520 + 12 right = 544
+    g = oracle.Geometry(3, 4, ALIGN_WRITE, 16, (520, 520, 520), (2, 4, 8))
+    assert (g.xdim, g.ydim, g.zdim, g.lo, g.ro) == (544, 520, 520, 12, 12)
+    assert g.elements == (512, 512, 512) and g.threads == (256, 128, 64)
+    # 16384x8192 interior, r=8, Read at 16: pitch 16400, no padding needed
+    g = oracle.Geometry(2, 8, ALIGN_READ, 16, (16400, 8208), (4, 16))
+    assert (g.xdim, g.ydim, g.lo, g.ro) == (16400, 8208, 0, 0)
+    # truncation to coarsening multiples (hip/benchmark.cpp:142-164)
+    g = oracle.Geometry(2, 1, ALIGN_NONE, 0, (11, 10), (4, 3))
+    assert g.elements == (8, 6, 1) and (g.xdim, g.ydim) == (10, 8)
+
+
+def test_geometry_matches_product_and_runtime():
+    """spec.geometry (generator), tcsg_geometry_compute (C runtime, no GPU
+    needed) and the oracle agree on random sizes."""
+    import random
+
+    from threadcoarsenedstencilgenerator_amd import runtime as rt
+    from threadcoarsenedstencilgenerator_amd import spec
+    rnd = random.Random(0)
+    for _ in range(300):
+        dim = rnd.choice((2, 3))
+        r = rnd.randint(1, 6)
+        C = (rnd.randint(1, 5), rnd.randint(1, 5), rnd.randint(1, 5) if dim == 3 else 1)
+        align = rnd.choice((ALIGN_NONE, ALIGN_READ, ALIGN_WRITE))
+        cl = 0 if align == ALIGN_NONE else rnd.choice((1, 3, 4, 8, 16))
+        size = tuple(rnd.randint(2 * r + 1, 2 * r + 40) for _ in range(dim))
+        o = oracle.Geometry(dim, r, align, cl, size, C)
+        d = spec.StencilDesc(dimension=dim, range=r, xc=C[0], yc=C[1], zc=C[2], alignment=align, cl_size=cl)
+        s = spec.geometry(d, size)
+        c = rt.geometry(d, size)
+        assert (o.xdim, o.ydim, o.zdim, o.lo, o.ro) == (s.xdim, s.ydim, s.zdim, s.lo, s.ro)
+        assert (c.size.x, c.size.y, c.size.z, c.left_offset, c.right_offset) == (o.xdim, o.ydim, o.zdim, o.lo,
+                                                                                   o.ro)
+        assert o.elements == s.elements == c.elements.tup()
+        assert o.threads == s.threads == c.threads.tup()
+        # alignment contract: Write -> first interior point starts a line; both -> pitch multiple of cl
+        if align != ALIGN_NONE:
+            assert o.xdim % cl == 0
+        if align == ALIGN_WRITE:
+            assert (o.lo + r) % cl == 0
+
+
+def test_step_writes_interior_only():
+    """Every interior point written, halo and padding untouched (sentinels)."""
+    for dim, align in itertools.product((2, 3), (ALIGN_NONE, ALIGN_READ, ALIGN_WRITE)):
+        g = _grid(dim, 2, align, 8 if align else 0)
+        a = np.zeros(g.n_points)
+        oracle.init_seeded(g, 0, 1, a)
+        out = np.full(g.n_points, -7.25)
+        oracle.step_f64(D(dim, BOX, ROW, JACOBI, 2), g, None, a, out)
+        inside = np.zeros(g.n_points, bool)
+        for (i, j, k) in _interior(g):
+            inside[_idx(g, i, j, k)] = True
+        assert np.all(out[~inside] == -7.25)
+        assert np.all(out[inside] != -7.25)
+
+
+def test_seeded_inputs():
+    g = _grid(3, 1)
+    a = np.zeros(g.n_points)
+    oracle.init_seeded(g, 0, 1, a)
+    inside = np.array([a[_idx(g, i, j, k)] for (i, j, k) in _interior(g)])
+    assert inside.min() >= 0.0 and inside.max() < 1.0 and len(set(inside.tolist())) == inside.size
+    w = oracle.mask_seeded(1, 25)
+    assert abs(np.abs(w).sum() - 1.0) < 1e-15 and w.min() < 0 < w.max()
+    # the product's mask generator is bit-identical
+    from threadcoarsenedstencilgenerator_amd import runtime as rt
+    assert np.array_equal(rt.mask_seeded(1, 25), w)
+    assert np.array_equal(rt.mask_seeded(12345, 289), oracle.mask_seeded(12345, 289))
+
+
+def _two_sum_exact(a, b):
+    s = np.float32(np.float32(a) + np.float32(b))
+    bp = np.float32(s - np.float32(a))
+    ap = np.float32(s - bp)
+    e = np.float32(np.float32(np.float32(a) - ap) + np.float32(np.float32(b) - bp))
+    return s, e
+
+
+def test_two_sum_is_error_free():
+    rng = np.random.default_rng(0)
+    for _ in range(2000):
+        a, b = np.float32(rng.standard_normal() * 10.0 ** rng.integers(-5, 5)), np.float32(rng.standard_normal())
+        s, e = _two_sum_exact(a, b)
+        assert Fraction(float(s)) + Fraction(float(e)) == Fraction(float(a)) + Fraction(float(b))
+
+
+def test_f32c_exact_on_small_integers():
+    """With integer inputs and a power-of-two Jacobi-free mask every FP32
+    operation is exact, so the compensated result equals the exact sum."""
+    g = _grid(3, 1, size=(8, 7, 7))
+    rng = np.random.default_rng(1)
+    a = rng.integers(-64, 64, g.n_points).astype(np.float32)
+    n = 7
+    mask = np.array([0.5, -0.25, 0.125, 1.0, 2.0, -0.5, 0.25])
+    out = np.zeros_like(a)
+    d = D(3, STAR, ROW, VARIABLE, 1, comp_group=2)
+    oracle.step_f32c(d, g, mask, a, out)
+    off, midx = oracle.offsets(3, STAR, ROW, 1)
+    for (i, j, k) in _interior(g):
+        s = sum(Fraction(float(a[_idx(g, i + int(o[0]), j + int(o[1]), k + int(o[2]))])) * Fraction(mask[m])
+                for o, m in zip(off, midx))
+        assert Fraction(float(out[_idx(g, i, j, k)])) == s
+
+
+@pytest.mark.parametrize("group", [1, 2, 5, 25])
+def test_f32c_close_to_f64(group):
+    """Compensated FP32 stays within a few FP32 ulps (normwise) of FP64."""
+    g = _grid(3, 4, ALIGN_WRITE, 16, size=(24, 20, 20))
+    a64 = np.zeros(g.n_points)
+    oracle.init_seeded(g, 2, 3, a64)
+    a32 = a64.astype(np.float32)
+    mask = oracle.mask_seeded(5, 25)
+    o64 = np.zeros_like(a64)
+    o32 = np.zeros_like(a32)
+    oracle.step_f64(D(3, STAR, ROW, VARIABLE, 4), g, mask, a64, o64)
+    oracle.step_f32c(D(3, STAR, ROW, VARIABLE, 4, comp_group=group), g, mask, a32, o32)
+    err = np.abs(o32.astype(np.float64) - o64).max() / np.abs(o64).max()
+    assert err < 4 * 2.0 ** -24, err
+
+
+def test_block_shapes_match_reference_primes():
+    """Sweep order and counts against the reference's own primes.cpp
+    (compiled from the reference tree into oracle/_ref when present), worked
+    examples otherwise."""
+    assert oracle.divisors(64) == [1, 2, 4, 8, 16, 32, 64]
+    assert oracle.divisors(192) == [1, 2, 4, 8, 16, 32, 64, 3, 6, 12, 24, 48, 96, 192]
+    assert len(oracle.block_shapes(2)) == 230
+    assert len(oracle.block_shapes(3)) == 1529
+    ref = oracle.ref_lib()
+    if ref is None:
+        pytest.skip("oracle/_ref not built (reference tree absent)")
+    import ctypes as C
+    for T in range(64, 1025, 64):
+        buf = np.zeros(3 * 20000, dtype=np.uint32)
+        n = ref.ref_prime_combinations(T, buf.ctypes.data_as(C.c_void_p), 20000)
+        assert oracle.divisors(T) == buf[:n].tolist()
+        n3 = ref.ref_prime_combinations3d(T, buf.ctypes.data_as(C.c_void_p), 20000)
+        mine = [s for s in oracle.block_shapes(3) if s[0] * s[1] * s[2] == T]
+        assert mine == [tuple(buf[3 * i:3 * i + 3].tolist()) for i in range(n3)]

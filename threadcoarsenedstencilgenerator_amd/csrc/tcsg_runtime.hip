@@ -177,6 +177,7 @@ __global__ void error_kernel(const float *__restrict__ a, const double *__restri
 
 int geometry_compute(const tcsg_desc &d, tcsg_dim3 size, tcsg_geometry &g)
 {
+    g = tcsg_geometry{};
     const uint32_t r = d.range;
     uint32_t sz[3] = {size.x, size.y, d.dimension == 3 ? size.z : 1u};
     const uint32_t c[3] = {d.xc, d.yc, d.zc};
@@ -357,15 +358,17 @@ int tcsg_check_block(const tcsg_kernel *k, tcsg_dim3 b)
     return TCSG_OK;
 }
 
-static int check_geom(const tcsg_kernel *k, const tcsg_geometry *g)
+static int check_geom(const tcsg_desc &d, const tcsg_geometry *g)
 {
-    const tcsg_desc &d = k->desc;
     if (g->dimension != d.dimension || g->range != d.range) return TCSG_ERR_ARG;
     tcsg_geometry ref;
     // the geometry must be one this kernel's coarsening and alignment produce
     tcsg_dim3 user{g->size.x - g->left_offset - g->right_offset, g->size.y, g->size.z};
     if (geometry_compute(d, user, ref) != TCSG_OK) return TCSG_ERR_ARG;
-    if (std::memcmp(&ref, g, sizeof(ref)) != 0) return TCSG_ERR_ARG;
+    auto eq3 = [](tcsg_dim3 a, tcsg_dim3 b) { return a.x == b.x && a.y == b.y && a.z == b.z; };
+    if (!eq3(ref.size, g->size) || !eq3(ref.elements, g->elements) || !eq3(ref.threads, g->threads) ||
+        ref.left_offset != g->left_offset || ref.right_offset != g->right_offset || ref.n_points != g->n_points)
+        return TCSG_ERR_ARG;
     if (d.variant == TCSG_STREAM) {
         const uint64_t es = elem_size(d);
         const uint64_t sx = (uint64_t)g->size.x * es, sxy = sx * g->size.y;
@@ -376,6 +379,12 @@ static int check_geom(const tcsg_kernel *k, const tcsg_geometry *g)
         if (bytes + span > kOobBase || span >= (1ull << 32) - kOobBase) return TCSG_ERR_TOO_LARGE;
     }
     return TCSG_OK;
+}
+
+int tcsg_geometry_check(const tcsg_desc *desc, const tcsg_geometry *g)
+{
+    if (!desc || !g || !desc_valid(*desc)) return TCSG_ERR_ARG;
+    return check_geom(*desc, g);
 }
 
 int tcsg_launch_init(tcsg_kernel *k, const tcsg_geometry *g, int mode, uint64_t seed, void *a, void *stream)
@@ -417,7 +426,7 @@ static int launch_one(tcsg_kernel *k, const tcsg_geometry *g, tcsg_dim3 block, t
 static int precheck(tcsg_kernel *k, const tcsg_geometry *g, tcsg_dim3 block, tcsg_dim3 blocks)
 {
     if (!k || !g) return TCSG_ERR_ARG;
-    int rc = check_geom(k, g);
+    int rc = check_geom(k->desc, g);
     if (rc) return rc;
     rc = tcsg_check_block(k, block);
     if (rc) return rc;

@@ -99,4 +99,6 @@ def default_build_set():
     descs += [cfg1_desc("plain"), cfg1_desc("stream"), cfg1_desc("dpp")]
     descs += cfg2_sweep() + cfg2_tuned()
     descs += [cfg4_desc(F64, "plain"), cfg4_desc(F64, "dpp")]
+    from . import tuned
+    descs += tuned.build_set()
     return list(dict.fromkeys(descs))

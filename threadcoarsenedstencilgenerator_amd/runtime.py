@@ -68,6 +68,7 @@ def _lib():
         L = C.CDLL(LIB)
         p, u32, i, u64 = C.c_void_p, C.c_uint32, C.c_int, C.c_uint64
         L.tcsg_geometry_compute.argtypes = [C.POINTER(CDesc), Dim3, C.POINTER(CGeometry)]
+        L.tcsg_geometry_check.argtypes = [C.POINTER(CDesc), C.POINTER(CGeometry)]
         L.tcsg_kernel_load.argtypes = [C.POINTER(p), C.c_char_p, C.POINTER(CDesc)]
         L.tcsg_kernel_free.argtypes = [p]
         L.tcsg_kernel_desc.argtypes = [p, C.POINTER(CDesc)]

@@ -1,0 +1,25 @@
+"""Best measured kernel configuration per benchmark workload on MI355X.
+
+Each entry is the code shape, coarsening and block shape that the block-shape
+and coarsening sweeps found fastest (the measurements are summarised under
+profiles/).  bench.py runs these.
+"""
+from __future__ import annotations
+
+from . import manifest
+from .spec import F32C, F64
+
+# 3D Star Variable r=4, 512^3 interior, FP64, Write alignment at 16 points
+CFG2_F64 = dict(desc=manifest.cfg2_desc(1, 4, 16, shape="dpp", max_threads=256), block=(64, 4, 1))
+# the same workload in FP32 storage with compensated accumulation
+CFG2_F32C = dict(desc=manifest.cfg2_desc(1, 4, 16, shape="dpp", max_threads=256, dtype=F32C), block=(64, 4, 1))
+
+
+def best(workload: str, dtype: int = F64):
+    if workload == "star3d_r4_512":
+        return CFG2_F32C if dtype == F32C else CFG2_F64
+    raise KeyError(workload)
+
+
+def build_set():
+    return [CFG2_F64["desc"], CFG2_F32C["desc"]]

@@ -10,9 +10,9 @@ from . import manifest
 from .spec import F32C, F64
 
 # 3D Star Variable r=4, 512^3 interior, FP64, Write alignment at 16 points
-CFG2_F64 = dict(desc=manifest.cfg2_desc(1, 4, 16, shape="dpp", max_threads=256), block=(64, 4, 1))
+CFG2_F64 = dict(desc=manifest.cfg2_desc(1, 4, 32, shape="dpp", max_threads=256), block=(64, 1, 1))
 # the same workload in FP32 storage with compensated accumulation
-CFG2_F32C = dict(desc=manifest.cfg2_desc(1, 4, 16, shape="dpp", max_threads=256, dtype=F32C), block=(64, 4, 1))
+CFG2_F32C = dict(desc=manifest.cfg2_desc(1, 4, 32, shape="dpp", max_threads=256, dtype=F32C), block=(64, 1, 1))
 
 
 def best(workload: str, dtype: int = F64):

@@ -19,9 +19,7 @@ def _id(case):
 
 
 def _blocks(d):
-    if d.dimension == 3:
-        return [(8, 4, 2), (64, 1, 2)]
-    return [(16, 4, 1), (64, 2, 1)]
+    return manifest.parity_blocks(d)
 
 
 @pytest.mark.parametrize("case", CASES, ids=[_id(c) for c in CASES])
@@ -39,7 +37,7 @@ F32 = manifest.f32c_cases()
 @pytest.mark.parametrize("case", F32, ids=[_id(c) for c in F32])
 def test_parity_f32c(case):
     d, size = case
-    block = (32, 2, 2) if d.dimension == 3 else (32, 4, 1)
+    block = manifest.parity_blocks(d)[0]
     print(f"case {d.key()} size {size} block {block}", flush=True)
     got, ref, _, _ = run_case(d, size, steps=3, block=block, mode=2)
     assert_bitwise(got, ref, d.key())

@@ -218,7 +218,7 @@ struct tcsg_kernel {
     void *mask = nullptr;
     bool mask_set = false;
     int legal_max = 0;
-    int xext_dpp = 0;     // STREAM kernel exchanging x-neighbours by DPP (params[16])
+    int xext_mode = 0;    // STREAM x-neighbour exchange (params[16]): 0 loads, 1 DPP, 2 wave-aligned DPP
 };
 
 extern "C" {
@@ -286,7 +286,8 @@ int tcsg_kernel_load(tcsg_kernel **out, const char *path, const tcsg_desc *expec
     e = hipFuncGetAttribute(&v, HIP_FUNC_ATTRIBUTE_MAX_THREADS_PER_BLOCK, k->fn);
     if (e != hipSuccess) { g_last_hip = (int)e; return fail(TCSG_ERR_HIP); }
     k->legal_max = std::min<int>(v, (int)q[15]);
-    k->xext_dpp = d.variant == TCSG_STREAM && q[16] != 0;
+    k->xext_mode = d.variant == TCSG_STREAM ? (int)q[16] : 0;
+    if (k->xext_mode > 2) return fail(TCSG_ERR_MISMATCH);
     *out = k;
     return TCSG_OK;
 }
@@ -342,7 +343,8 @@ int tcsg_check_block(const tcsg_kernel *k, tcsg_dim3 b)
     if (!b.x || !b.y || !b.z || b.x > 1024 || b.y > 1024 || b.z > 1024) return TCSG_ERR_BLOCK_SHAPE;
     const uint64_t t = (uint64_t)b.x * b.y * b.z;
     if (t > (uint64_t)k->legal_max) return TCSG_ERR_BLOCK_SHAPE;
-    if (k->xext_dpp) {
+    if (k->xext_mode == 2 && b.x % 64 != 0) return TCSG_ERR_BLOCK_SHAPE;  // block rows must be whole waves
+    if (k->xext_mode == 1) {
         // every run of lanes in one block row and one wave must span the
         // H = ceil(r/xc) lanes the x-neighbour exchange reaches
         const uint32_t H = (k->desc.range + k->desc.xc - 1) / k->desc.xc;

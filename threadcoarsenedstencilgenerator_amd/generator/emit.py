@@ -180,14 +180,18 @@ class _Emitter:
             w("  e = (a - ap) + (b - bp);")
             w("  s = x;")
             w("}")
-        if d.variant == STREAM and d.xext == "dpp":
-            # lanes without a source lane (the wave's ends) receive 0; the kernel
-            # never uses those values
+        if d.variant == STREAM and d.xext in ("dpp", "dppw"):
+            # tcsg_shr1/shl1: lane i receives lane i-1 / i+1 (0 where no such lane).
+            # tcsg_shr1_or/shl1_or: the same, but lanes without a source lane keep `old`.
             for nm, ctrl in (("shr1", DPP_WAVE_SHR1), ("shl1", DPP_WAVE_SHL1)):
                 if self.f32:
                     w(f"__device__ __forceinline__ float tcsg_{nm}(float v) {{")
                     w(f"  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp("
                       f"__builtin_bit_cast(int, v), {ctrl:#x}, 0xf, 0xf, true));")
+                    w("}")
+                    w(f"__device__ __forceinline__ float tcsg_{nm}_or(float old, float v) {{")
+                    w(f"  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, old), "
+                      f"__builtin_bit_cast(int, v), {ctrl:#x}, 0xf, 0xf, false));")
                     w("}")
                 else:
                     w(f"__device__ __forceinline__ double tcsg_{nm}(double v) {{")
@@ -195,15 +199,24 @@ class _Emitter:
                     w(f"  return __hiloint2double(__builtin_amdgcn_mov_dpp(hi, {ctrl:#x}, 0xf, 0xf, true),")
                     w(f"                          __builtin_amdgcn_mov_dpp(lo, {ctrl:#x}, 0xf, 0xf, true));")
                     w("}")
+                    w(f"__device__ __forceinline__ double tcsg_{nm}_or(double old, double v) {{")
+                    w("  const int lo = __double2loint(v), hi = __double2hiint(v);")
+                    w("  const int olo = __double2loint(old), ohi = __double2hiint(old);")
+                    w(f"  return __hiloint2double(__builtin_amdgcn_update_dpp(ohi, hi, {ctrl:#x}, 0xf, 0xf, false),")
+                    w(f"                          __builtin_amdgcn_update_dpp(olo, lo, {ctrl:#x}, 0xf, 0xf, false));")
+                    w("}")
         if d.variant == STREAM:
             bits = 32 if self.f32 else 64
             ity = "int" if self.f32 else "__attribute__((ext_vector_type(2))) int"
-            w(f"__device__ __forceinline__ {T} tcsg_ld(__amdgpu_buffer_rsrc_t r, uint32_t off) {{")
-            w(f"  return __builtin_bit_cast({T}, __builtin_amdgcn_raw_buffer_load_b{bits}(r, off, 0, 0));")
+            w("// Buffer access: per-lane byte offset `off` (range-checked: lanes given an")
+            w("// offset past the buffer read 0 and their stores are dropped) plus a")
+            w("// wave-uniform non-negative row/plane offset `soff` in an SGPR.")
+            w(f"__device__ __forceinline__ {T} tcsg_ld(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff) {{")
+            w(f"  return __builtin_bit_cast({T}, __builtin_amdgcn_raw_buffer_load_b{bits}(r, off, soff, 0));")
             w("}")
             aux = 2 if d.nt_store else 0
-            w(f"__device__ __forceinline__ void tcsg_st(__amdgpu_buffer_rsrc_t r, uint32_t off, {T} v) {{")
-            w(f"  __builtin_amdgcn_raw_buffer_store_b{bits}(__builtin_bit_cast({ity}, v), r, off, 0, {aux});")
+            w(f"__device__ __forceinline__ void tcsg_st(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff, {T} v) {{")
+            w(f"  __builtin_amdgcn_raw_buffer_store_b{bits}(__builtin_bit_cast({ity}, v), r, off, soff, {aux});")
             w("}")
         w("")
 
@@ -301,8 +314,11 @@ class _Emitter:
         R, LO = self.R, d.left_offset
         XC = self.XC
         es = 4 if self.f32 else 8
-        dpp = d.xext == "dpp"
+        dpp = d.xext in ("dpp", "dppw")
+        wave_aligned = d.xext == "dppw"
         H = self.halo_lanes()
+        ZR = R if self.three else 0
+        XL = 0 if dpp else R  # columns left of x0 covered by the per-lane base
         self.thread_counts()
         self.block_coords()
         w("  // inrow: the thread's row/plane is inside the interior; such lanes load (and")
@@ -311,11 +327,12 @@ class _Emitter:
         w("  const bool inrow = gy < thy && gz < thz;")
         w("  const bool valid = inrow && gx < thx;")
         w(f"  const uint32_t x0 = gx * {XC}u + {R + LO}u;")
-        w(f"  const uint32_t y0 = gy * {self.YC}u + {R}u;")
-        w(f"  const uint32_t z0 = gz * {self.ZC}u + {R}u;" if self.three else "  const uint32_t z0 = 0u;")
+        w(f"  const uint32_t y0 = gy * {self.YC}u;            // first row of the footprint")
+        w(f"  const uint32_t z0 = gz * {self.ZC}u;            // first plane of the footprint" if self.three
+          else "  const uint32_t z0 = 0u;")
         w(f"  const uint32_t sx = xdim * {es}u;            // row pitch in bytes")
         w(f"  const uint32_t sxy = xdim * ydim * {es}u;     // plane pitch in bytes")
-        w(f"  const uint32_t real = (z0 * ydim + y0) * xdim * {es}u + x0 * {es}u;")
+        w(f"  const uint32_t real = (z0 * ydim + y0) * xdim * {es}u + (x0 - {XL}u) * {es}u;")
         w(f"  // offsets past the buffer: loads read 0, stores are dropped by the range check")
         w(f"  const uint32_t base = inrow ? real : {OOB_BASE:#x}u;")
         w(f"  const uint32_t sbase = valid ? real : {OOB_BASE:#x}u;")
@@ -323,42 +340,37 @@ class _Emitter:
         w("  const __amdgpu_buffer_rsrc_t rout = __builtin_amdgcn_make_buffer_rsrc((void*)a.out, (short)0, (int)a.out_bytes, 0x00020000);")
         self.weights()
         if dpp:
-            w("  // Lane segments: runs of lanes in one block row and one wave.  tL/tR are the")
-            w("  // lane's distances to its segment's ends; the runtime only launches block")
-            w(f"  // shapes whose segments are at least H = {H} lanes long.")
             w("  const uint32_t lid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);")
             w("  const uint32_t lane = lid & 63u;")
-            w("  const uint32_t tx = threadIdx.x;")
-            w("  const uint32_t tL = min(tx, lane);")
-            w("  const uint32_t tR = min(blockDim.x - 1u - tx, 63u - lane);")
-            w(f"  const uint32_t eoffL = tL < {H}u ? base - {H * XC * es}u : {OOB_BASE:#x}u;")
-            w(f"  const uint32_t eoffR = tR < {H}u ? base + {H * XC * es}u : {OOB_BASE:#x}u;")
+            if wave_aligned:
+                w("  // Block rows are whole waves (the runtime only launches block x extents")
+                w(f"  // that are multiples of 64): the {H} edge lanes of each wave load the")
+                w("  // columns beyond the wave, which enter the lane-shift chains as the DPP")
+                w("  // `old` value of lanes that have no source lane.")
+                w(f"  const bool edgeL = lane < {H}u, edgeR = lane + {H}u > 63u;")
+            else:
+                w("  // Lane segments: runs of lanes in one block row and one wave.  tL/tR are the")
+                w("  // lane's distances to its segment's ends; the runtime only launches block")
+                w(f"  // shapes whose segments are at least H = {H} lanes long.")
+                w("  const uint32_t tx = threadIdx.x;")
+                w("  const uint32_t tL = min(tx, lane);")
+                w("  const uint32_t tR = min(blockDim.x - 1u - tx, 63u - lane);")
+                w(f"  const bool edgeL = tL < {H}u, edgeR = tR < {H}u;")
+        if dpp:
+            w("  // Extra columns of the edge lanes; the left ones are addressed from their own")
+            w("  // offsets so that no negative immediate is folded into a load.")
+            for ip in range(XC):
+                w(f"  const uint32_t eoffL{ip} = edgeL ? base - {(H * XC - ip) * es}u : {OOB_BASE:#x}u;")
+            w(f"  const uint32_t eoffR = edgeR ? base + {H * XC * es}u : {OOB_BASE:#x}u;")
         pts, contrib = self.footprint()
         last = {}
         for idx, p in enumerate(pts):
             for (o, t, m) in contrib[p]:
                 last[o] = idx
-        # acquisition groups: Row -> rows (z, y); Column -> planes (z)
-        if d.orientation == ROW:
-            gkey = lambda p: (p[2], p[1])
-        else:
-            gkey = lambda p: (p[2],)
-        groups = []
-        for idx, p in enumerate(pts):
-            if not groups or groups[-1][0] != gkey(p):
-                groups.append((gkey(p), []))
-            groups[-1][1].append((idx, p))
-        declared = set()
 
-        def rowoff(py, pz):
-            return f"(uint32_t)({py}) * sx + (uint32_t)({pz}) * sxy"
-
-        def rd(py, pz, prefix="d", b="base"):
-            k = f"{prefix}_{_n(py)}_{_n(pz)}"
-            if k not in declared:
-                declared.add(k)
-                w(f"  const uint32_t {k} = {b} + {rowoff(py, pz)};")
-            return k
+        def soff(py, pz):
+            """Uniform row/plane byte offset of thread-relative row (py, pz)."""
+            return f"({py + R}u * sx + {pz + ZR}u * sxy)"
 
         def vname(px, py, pz):
             return f"v_{_n(px)}_{_n(py)}_{_n(pz)}"
@@ -371,67 +383,113 @@ class _Emitter:
             h = px // XC
             return "R", h, px - h * XC
 
-        for _, members in groups:
-            gpts = [p for _, p in members]
-            need_own = set()
-            for (px, py, pz) in gpts:
+        # streaming groups: planes in 3D, rows in 2D Row, the whole tile in 2D Column
+        if self.three:
+            skey = lambda p: p[2]
+        elif d.orientation == ROW:
+            skey = lambda p: p[1]
+        else:
+            skey = lambda p: 0
+        order = []
+        members = defaultdict(list)
+        for idx, p in enumerate(pts):
+            k = skey(p)
+            if k not in members:
+                order.append(k)
+            members[k].append((idx, p))
+
+        def loads_of(k):
+            """(name, offset expr, soff expr) of the loads plane/row k needs."""
+            out = []
+            seen = set()
+            for _, (px, py, pz) in members[k]:
                 if 0 <= px < XC or not dpp:
-                    need_own.add((px, py, pz))
+                    srcs = [(px, py, pz)]
                 else:
-                    need_own.add((hop(px)[2], py, pz))
-            for (px, py, pz) in sorted(need_own, key=lambda p: spec.sort_key(d, *p)):
-                nm = vname(px, py, pz)
+                    side, h, ip = hop(px)
+                    srcs = [(ip, py, pz), ("E" + side, ip, py, pz)]
+                for sp in srcs:
+                    if sp in seen:
+                        continue
+                    seen.add(sp)
+                    if sp[0] in ("EL", "ER"):
+                        _, ip, py2, pz2 = sp
+                        side = sp[0][1]
+                        off = f"eoffL{ip}" if side == "L" else f"eoffR + {ip * es}u"
+                        out.append((f"e{side}0_{ip}_{_n(py2)}_{_n(pz2)}", off, soff(py2, pz2)))
+                    else:
+                        qx, qy, qz = sp
+                        out.append((vname(qx, qy, qz), f"base + {(qx + XL) * es}u", soff(qy, qz)))
+            return out
+
+        declared = set()
+
+        def emit_loads(k):
+            for nm, off, so in loads_of(k):
                 if nm in declared:
                     continue
                 declared.add(nm)
-                dk = rd(py, pz)
-                w(f"  const {T} {nm} = tcsg_ld(rin, {dk} + {px * es}u);" if px >= 0 else
-                  f"  const {T} {nm} = tcsg_ld(rin, {dk} - {-px * es}u);")
+                w(f"  const {T} {nm} = tcsg_ld(rin, {off}, {so});")
+
+        D = d.prefetch
+        for i in range(min(D, len(order))):
+            emit_loads(order[i])
+        for gi, k in enumerate(order):
+            if gi + D < len(order):
+                emit_loads(order[gi + D])
+            emit_loads(k)
+            gpts = [p for _, p in members[k]]
             if dpp:
+                # x-neighbours: main lane-shift chain from the own columns, halo
+                # chain from the edge lanes' extra loads
                 for (px, py, pz) in gpts:
                     nm = vname(px, py, pz)
                     if 0 <= px < XC or nm in declared:
                         continue
                     side, h, ip = hop(px)
                     main_fn, e_fn = ("tcsg_shr1", "tcsg_shl1") if side == "L" else ("tcsg_shl1", "tcsg_shr1")
-                    # main chain: value of lane t-h (left) / t+h (right)
-                    prev = vname(ip, py, pz)
-                    for hh in range(1, h + 1):
-                        sn = f"s{side}{hh}_{ip}_{_n(py)}_{_n(pz)}"
-                        if sn not in declared:
-                            declared.add(sn)
-                            w(f"  const {T} {sn} = {main_fn}({prev});")
-                        prev = sn
-                    # halo chain: extra columns loaded by the H edge lanes of each segment
                     en = f"e{side}0_{ip}_{_n(py)}_{_n(pz)}"
-                    if en not in declared:
-                        declared.add(en)
-                        ek = f"eoff{side}"
-                        w(f"  const {T} {en} = tcsg_ld(rin, {ek} + {rowoff(py, pz)} + {ip * es}u);")
-                    eprev = en
-                    for j in range(1, H - h + 1):
-                        gn = f"e{side}{j}_{ip}_{_n(py)}_{_n(pz)}"
-                        if gn not in declared:
-                            declared.add(gn)
-                            w(f"  const {T} {gn} = {e_fn}({eprev});")
-                        eprev = gn
-                    w(f"  const {T} {nm} = t{side} >= {h}u ? {prev} : {eprev};")
+                    gname = lambda j: f"e{side}{j}_{ip}_{_n(py)}_{_n(pz)}"
+                    for j in range(1, H):
+                        if gname(j) not in declared:
+                            declared.add(gname(j))
+                            w(f"  const {T} {gname(j)} = {e_fn}({gname(j - 1)});")
+                    if wave_aligned:
+                        prev = vname(ip, py, pz)
+                        for hh in range(1, h + 1):
+                            sn = f"m{side}{hh}_{ip}_{_n(py)}_{_n(pz)}"
+                            if sn not in declared:
+                                declared.add(sn)
+                                w(f"  const {T} {sn} = {main_fn}_or({gname(H - hh)}, {prev});")
+                            prev = sn
+                        w(f"  const {T} {nm} = {prev};")
+                    else:
+                        prev = vname(ip, py, pz)
+                        for hh in range(1, h + 1):
+                            sn = f"s{side}{hh}_{ip}_{_n(py)}_{_n(pz)}"
+                            if sn not in declared:
+                                declared.add(sn)
+                                w(f"  const {T} {sn} = {main_fn}({prev});")
+                            prev = sn
+                        w(f"  const {T} {nm} = t{side} >= {h}u ? {prev} : {gname(H - h)};")
                     declared.add(nm)
-            for idx, p in members:
+            for idx, p in members[k]:
                 v = vname(*p)
                 for (o, t, m) in sorted(contrib[p], key=lambda c: c[0]):
                     w("  " + self.acc_stmt(o, t, m, v))
                 for o in self.outputs():
                     if last[o] == idx:
-                        i, j, k = o
-                        sk = rd(j, k, "sd", "sbase")
-                        w(f"  tcsg_st(rout, {sk} + {i * es}u, {self.final_expr(o)});")
+                        i, j, kk = o
+                        w(f"  tcsg_st(rout, sbase + {(i + XL) * es}u, {soff(j, kk)}, {self.final_expr(o)});")
 
     # ------------------------------------------------------------------ all
     def source(self) -> str:
         d = self.d
         self.preamble()
-        self.w(f'extern "C" __global__ void __launch_bounds__({d.max_threads}) tcsg_update(const tcsg_kargs a)')
+        attr = ""
+        if d.variant == STREAM and d.waves_per_eu:
+            attr = f" __attribute__((amdgpu_waves_per_eu({d.waves_per_eu})))"
+        self.w(f'extern "C" __global__ void __launch_bounds__({d.max_threads}){attr} tcsg_update(const tcsg_kargs a)')
         self.w("{")
         if d.variant == PLAIN:
             self.body_plain()

@@ -16,7 +16,17 @@ SHAPES = {
     "plain": dict(variant=PLAIN),
     "stream": dict(variant=STREAM, xext="load", xcd_remap=True, nt_store=True),
     "dpp": dict(variant=STREAM, xext="dpp", xcd_remap=True, nt_store=True),
+    "dppw": dict(variant=STREAM, xext="dppw", xcd_remap=True, nt_store=True),
 }
+# block shapes each code shape is tested with (dppw needs x extents that are multiples of 64)
+PARITY_BLOCKS = {
+    2: {"any": [(16, 4, 1), (64, 2, 1)], "dppw": [(64, 2, 1), (128, 1, 1)]},
+    3: {"any": [(8, 4, 2), (64, 1, 2)], "dppw": [(64, 1, 2), (128, 2, 1)]},
+}
+
+
+def parity_blocks(desc):
+    return PARITY_BLOCKS[desc.dimension]["dppw" if desc.xext == "dppw" else "any"]
 
 
 def parity_cases():
@@ -31,6 +41,8 @@ def parity_cases():
             ]
             if nb == STAR:
                 pts.append(dict(range=3, xc=1, yc=2, zc=3, alignment=ALIGN_READ, cl_size=8))
+                # edge-lane columns left of the first interior column (r=3, xc=2: H*xc > r)
+                pts.append(dict(range=3, xc=2, yc=1, zc=2, alignment=ALIGN_NONE, cl_size=0, prefetch=2))
             for p in pts:
                 if dim == 2:
                     p = dict(p, zc=1)
@@ -44,7 +56,7 @@ def f32c_cases():
     out = []
     for dim, nb, coeff, orient in ((3, STAR, VARIABLE, ROW), (2, BOX, VARIABLE, COLUMN), (3, BOX, JACOBI, ROW),
                                    (2, STAR, JACOBI, ROW)):
-        for shape in ("plain", "dpp"):
+        for shape in ("plain", "dpp", "dppw"):
             d = StencilDesc(dimension=dim, neighbourhood=nb, orientation=orient, coeff=coeff, range=2, xc=2,
                             yc=2, zc=2 if dim == 3 else 1, alignment=ALIGN_WRITE, cl_size=16, dtype=F32C,
                             **SHAPES[shape])
@@ -78,6 +90,10 @@ def cfg2_tuned():
                       (2, 2, 32), (1, 2, 32)):
         for mt in (256, 512):
             out.append(cfg2_desc(x, y, z, shape="dpp", max_threads=mt))
+    for (x, y, z) in ((1, 4, 32), (1, 4, 16), (1, 2, 32), (2, 2, 32), (1, 3, 32), (1, 4, 64), (2, 4, 32),
+                      (1, 6, 32), (1, 8, 32)):
+        for pf, wpe in ((0, 0), (1, 0), (2, 0), (1, 3), (1, 4)):
+            out.append(cfg2_desc(x, y, z, shape="dppw", max_threads=256).replace(prefetch=pf, waves_per_eu=wpe))
     return out
 
 

@@ -35,8 +35,9 @@ class StencilDesc:
 
     dimension/neighbourhood/orientation/coeff/range/coarsening/alignment/
     cl_size mirror the reference generator argv.  dtype, variant and the
-    tuning knobs (max_threads, xext, xcd_remap, nt_store, comp_group) have no
-    reference counterpart: they select the MI355X code shape.
+    tuning knobs (max_threads, xext, xcd_remap, nt_store, comp_group,
+    prefetch, waves_per_eu) have no reference counterpart: they select the
+    MI355X code shape.
     """
 
     dimension: int = 3
@@ -52,10 +53,14 @@ class StencilDesc:
     dtype: int = F64
     variant: int = PLAIN
     max_threads: int = 1024
-    xext: str = "load"       # "load" | "dpp": how x-neighbours outside the thread tile are obtained
+    xext: str = "load"       # how x-neighbours outside the thread's columns are obtained:
+    #                          "load" global loads, "dpp" lane shifts (any block shape),
+    #                          "dppw" lane shifts for blocks whose x extent is a multiple of 64
     xcd_remap: bool = False  # remap block ids so neighbouring tiles share an XCD's L2
     nt_store: bool = False   # nontemporal stores of the outputs
     comp_group: int = 5      # F32C: terms summed per group before the TwoSum combine
+    prefetch: int = 1        # STREAM: planes whose loads are issued ahead of the arithmetic
+    waves_per_eu: int = 0    # STREAM: occupancy the compiler must allow (0: its own choice)
 
     def __post_init__(self):
         if self.dimension not in (2, 3):
@@ -76,8 +81,10 @@ class StencilDesc:
             raise ValueError("unknown alignment")
         if self.alignment != ALIGN_NONE and self.cl_size <= 0:
             raise ValueError("size of alignment has to be > 0")
-        if self.xext not in ("load", "dpp"):
-            raise ValueError("xext must be 'load' or 'dpp'")
+        if self.xext not in ("load", "dpp", "dppw"):
+            raise ValueError("xext must be 'load', 'dpp' or 'dppw'")
+        if self.prefetch < 0 or self.waves_per_eu < 0 or self.waves_per_eu > 8:
+            raise ValueError("prefetch must be >= 0 and waves_per_eu in [0, 8]")
         if self.max_threads % 64 or not 64 <= self.max_threads <= 1024:
             raise ValueError("max_threads must be a multiple of 64 in [64, 1024]")
         if self.comp_group < 1:
@@ -126,8 +133,8 @@ class StencilDesc:
         return [0x54435347, 1, self.dimension, self.neighbourhood, self.orientation,
                 self.coeff, self.range, self.xc, self.yc, self.zc, self.alignment,
                 self.cl_size, self.dtype, self.variant, self.n, self.max_threads,
-                int(self.xext == "dpp"), int(self.xcd_remap), int(self.nt_store),
-                self.comp_group, 0, 0, 0, 0]
+                {"load": 0, "dpp": 1, "dppw": 2}[self.xext], int(self.xcd_remap), int(self.nt_store),
+                self.comp_group, self.prefetch, self.waves_per_eu, 0, 0]
 
     def replace(self, **kw) -> "StencilDesc":
         return dataclasses.replace(self, **kw)

@@ -233,7 +233,10 @@ class _Emitter:
             w("    const uint32_t q = nblk >> 3, rm = nblk & 7u, xcd = lin & 7u, k = lin >> 3;")
             w("    lin = xcd < rm ? xcd * (q + 1u) + k : rm * (q + 1u) + (xcd - rm) * q + k;")
             w("  }")
-            w("  const uint32_t bxi = lin % nbx, byi = (lin / nbx) % nby, bzi = lin / (nbx * nby);")
+            if self.d.zfirst:
+                w("  const uint32_t bzi = lin % nbz, byi = (lin / nbz) % nby, bxi = lin / (nbz * nby);")
+            else:
+                w("  const uint32_t bxi = lin % nbx, byi = (lin / nbx) % nby, bzi = lin / (nbx * nby);")
         else:
             w("  const uint32_t bxi = blockIdx.x, byi = blockIdx.y, bzi = blockIdx.z;")
         w("  const uint32_t gx = bxi * blockDim.x + threadIdx.x;")

@@ -16,7 +16,7 @@ SHAPES = {
     "plain": dict(variant=PLAIN),
     "stream": dict(variant=STREAM, xext="load", xcd_remap=True, nt_store=True),
     "dpp": dict(variant=STREAM, xext="dpp", xcd_remap=True, nt_store=True),
-    "dppw": dict(variant=STREAM, xext="dppw", xcd_remap=True, nt_store=True),
+    "dppw": dict(variant=STREAM, xext="dppw", xcd_remap=True, nt_store=True, zfirst=True),
 }
 # block shapes each code shape is tested with (dppw needs x extents that are multiples of 64)
 PARITY_BLOCKS = {
@@ -94,6 +94,10 @@ def cfg2_tuned():
                       (1, 6, 32), (1, 8, 32)):
         for pf, wpe in ((0, 0), (1, 0), (2, 0), (1, 3), (1, 4)):
             out.append(cfg2_desc(x, y, z, shape="dppw", max_threads=256).replace(prefetch=pf, waves_per_eu=wpe))
+    for (x, y, z) in ((2, 4, 32), (2, 4, 64), (2, 4, 128), (2, 2, 64), (2, 2, 128), (1, 4, 64), (1, 4, 128),
+                      (1, 8, 64), (2, 3, 64), (2, 3, 128), (4, 2, 64), (1, 6, 64)):
+        for zf in (False, True):
+            out.append(cfg2_desc(x, y, z, shape="dppw", max_threads=256).replace(prefetch=0, zfirst=zf))
     return out
 
 

@@ -36,7 +36,7 @@ class StencilDesc:
     dimension/neighbourhood/orientation/coeff/range/coarsening/alignment/
     cl_size mirror the reference generator argv.  dtype, variant and the
     tuning knobs (max_threads, xext, xcd_remap, nt_store, comp_group,
-    prefetch, waves_per_eu) have no reference counterpart: they select the
+    prefetch, waves_per_eu, zfirst) have no reference counterpart: they select the
     MI355X code shape.
     """
 
@@ -61,6 +61,8 @@ class StencilDesc:
     comp_group: int = 5      # F32C: terms summed per group before the TwoSum combine
     prefetch: int = 1        # STREAM: planes whose loads are issued ahead of the arithmetic
     waves_per_eu: int = 0    # STREAM: occupancy the compiler must allow (0: its own choice)
+    zfirst: bool = False     # with xcd_remap: order tiles z-fastest, so z-neighbour tiles (which
+    #                          share halo planes) run together on one XCD
 
     def __post_init__(self):
         if self.dimension not in (2, 3):
@@ -134,7 +136,7 @@ class StencilDesc:
                 self.coeff, self.range, self.xc, self.yc, self.zc, self.alignment,
                 self.cl_size, self.dtype, self.variant, self.n, self.max_threads,
                 {"load": 0, "dpp": 1, "dppw": 2}[self.xext], int(self.xcd_remap), int(self.nt_store),
-                self.comp_group, self.prefetch, self.waves_per_eu, 0, 0]
+                self.comp_group, self.prefetch, self.waves_per_eu, int(self.zfirst), 0]
 
     def replace(self, **kw) -> "StencilDesc":
         return dataclasses.replace(self, **kw)

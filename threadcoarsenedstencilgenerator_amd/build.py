@@ -45,20 +45,27 @@ def kernel_path(desc: spec.StencilDesc) -> str:
 
 
 def build_kernel(desc: spec.StencilDesc, force: bool = True) -> str:
-    """Emit and compile one kernel; returns the code-object path."""
+    """Emit and compile one kernel; returns the code-object path.  The
+    generated source is compiled from a temporary file; a .sha1 sidecar
+    records which source the code object was built from."""
+    import hashlib
+    import tempfile
     os.makedirs(KDIR, exist_ok=True)
     co = kernel_path(desc)
-    src = co[:-3] + ".hip"
     text = emit_source(desc)
-    if not force and os.path.exists(co) and os.path.exists(src) and open(src).read() == text:
+    digest = hashlib.sha1(text.encode()).hexdigest()
+    side = co[:-3] + ".sha1"
+    if not force and os.path.exists(co) and os.path.exists(side) and open(side).read().strip() == digest:
         return co
-    tmp_src = src + f".{os.getpid()}.tmp"
-    with open(tmp_src, "w") as f:
-        f.write(text)
-    os.replace(tmp_src, src)
-    tmp = co + f".{os.getpid()}.tmp"
-    _run([HIPCC, *HIP_FLAGS, "--genco", "--no-gpu-bundle-output", "-x", "hip", src, "-o", tmp])
+    with tempfile.TemporaryDirectory(prefix="tcsg_") as td:
+        src = os.path.join(td, desc.key() + ".hip")
+        with open(src, "w") as f:
+            f.write(text)
+        tmp = co + f".{os.getpid()}.tmp"
+        _run([HIPCC, *HIP_FLAGS, "--genco", "--no-gpu-bundle-output", "-x", "hip", src, "-o", tmp])
     os.replace(tmp, co)
+    with open(side, "w") as f:
+        f.write(digest + "\n")
     return co
 
 

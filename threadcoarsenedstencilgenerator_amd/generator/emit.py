@@ -72,8 +72,9 @@ def _hexf(x: float, f32: bool) -> str:
 
 
 class _Emitter:
-    def __init__(self, d: StencilDesc):
+    def __init__(self, d: StencilDesc, target: str = "gfx950"):
         self.d = d
+        self.target = target
         self.f32 = d.dtype == F32C
         self.T = "float" if self.f32 else "double"
         self.R = d.range
@@ -114,24 +115,24 @@ class _Emitter:
         n = self.d.n
         if not self.f32:
             if t == 0:
-                return f"{self.T} {a} = {v} * {self.wname(m)};" if var else f"{self.T} {a} = {v};"
-            return f"{a} = __builtin_fma({v}, {self.wname(m)}, {a});" if var else f"{a} = {a} + {v};"
+                return f"TV {a} = {v} * {self.wname(m)};" if var else f"TV {a} = {v};"
+            return f"{a} = tcsg_fma({v}, {self.wname(m)}, {a});" if var else f"{a} = {a} + {v};"
         # F32C: groups of g terms summed with one FMA chain, groups combined by TwoSum
         g = self.d.comp_group
         gi = t // g
         q, s, c = f"q{gi}_{i}_{j}_{k}", f"s_{i}_{j}_{k}", f"c_{i}_{j}_{k}"
         out = []
         if t % g == 0:
-            out.append(f"float {q} = {v} * {self.wname(m)};" if var else f"float {q} = {v};")
+            out.append(f"TV {q} = {v} * {self.wname(m)};" if var else f"TV {q} = {v};")
         else:
-            out.append(f"{q} = __builtin_fmaf({v}, {self.wname(m)}, {q});" if var else f"{q} = {q} + {v};")
+            out.append(f"{q} = tcsg_fma({v}, {self.wname(m)}, {q});" if var else f"{q} = {q} + {v};")
         if t % g == g - 1 or t == n - 1:
             if gi == 0:
-                out.append(f"float {s} = {q};")
+                out.append(f"TV {s} = {q};")
             elif gi == 1:
-                out.append(f"float {c}; tcsg_two_sum({s}, {q}, {s}, {c});")
+                out.append(f"TV {c}; tcsg_two_sum({s}, {q}, {s}, {c});")
             else:
-                out.append(f"{{ float e_; tcsg_two_sum({s}, {q}, {s}, e_); {c} = {c} + e_; }}")
+                out.append(f"{{ TV e_; tcsg_two_sum({s}, {q}, {s}, e_); {c} = {c} + e_; }}")
         return " ".join(out)
 
     def final_expr(self, o):
@@ -151,6 +152,7 @@ class _Emitter:
         d = self.d
         T = self.T
         w = self.w
+        cpu = self.target == "cpu"
         w("// Generated by threadcoarsenedstencilgenerator_amd.generator -- do not edit.")
         nbname = "Box" if d.neighbourhood == BOX else "Star"
         w(f"// {d.dimension}D {nbname} r={d.range} "
@@ -158,9 +160,17 @@ class _Emitter:
           f"{'Row' if d.orientation == ROW else 'Column'} coarsening {d.xc}x{d.yc}x{d.zc} "
           f"alignment {['None', 'Read', 'Write'][d.alignment]}/{d.cl_size} "
           f"{'f32c' if self.f32 else 'f64'} {'stream' if d.variant == STREAM else 'plain'} "
-          f"xext={d.xext} xcd_remap={int(d.xcd_remap)} nt={int(d.nt_store)}")
-        w("#include <hip/hip_runtime.h>")
-        w("#include <stdint.h>")
+          f"xext={d.xext} xcd_remap={int(d.xcd_remap)} nt={int(d.nt_store)}"
+          + (" (CPU emulation build)" if cpu else ""))
+        if cpu:
+            w('#include "cpu_emu.h"')
+            w(f"typedef Lane<{T}> TV;")
+        else:
+            w("#include <hip/hip_runtime.h>")
+            w("#include <stdint.h>")
+            w("typedef uint32_t u32;")
+            w("typedef bool lbool;")
+            w(f"typedef {T} TV;")
         w("")
         w("struct tcsg_kargs {")
         w(f"  const {T}* in;")
@@ -172,15 +182,20 @@ class _Emitter:
         blob = ", ".join(str(v) for v in d.params_blob())
         w(f'extern "C" __device__ const uint32_t tcsg_params[24] = {{{blob}}};')
         w("")
+        if not cpu:
+            w("template <class V> __device__ __forceinline__ V tcsg_sel(bool c, V a, V b) { return c ? a : b; }")
+            w("__device__ __forceinline__ uint32_t tcsg_min(uint32_t a, uint32_t b) { return a < b ? a : b; }")
+            w("__device__ __forceinline__ double tcsg_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }")
+            w("__device__ __forceinline__ float tcsg_fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }")
         if self.f32:
-            w("__device__ __forceinline__ void tcsg_two_sum(float a, float b, float& s, float& e) {")
-            w("  const float x = a + b;")
-            w("  const float bp = x - a;")
-            w("  const float ap = x - bp;")
+            w("template <class F> __device__ __forceinline__ void tcsg_two_sum(F a, F b, F& s, F& e) {")
+            w("  const F x = a + b;")
+            w("  const F bp = x - a;")
+            w("  const F ap = x - bp;")
             w("  e = (a - ap) + (b - bp);")
             w("  s = x;")
             w("}")
-        if d.variant == STREAM and d.xext in ("dpp", "dppw"):
+        if d.variant == STREAM and d.xext in ("dpp", "dppw") and not cpu:
             # tcsg_shr1/shl1: lane i receives lane i-1 / i+1 (0 where no such lane).
             # tcsg_shr1_or/shl1_or: the same, but lanes without a source lane keep `old`.
             for nm, ctrl in (("shr1", DPP_WAVE_SHR1), ("shl1", DPP_WAVE_SHL1)):
@@ -206,19 +221,52 @@ class _Emitter:
                     w(f"                          __builtin_amdgcn_update_dpp(olo, lo, {ctrl:#x}, 0xf, 0xf, false));")
                     w("}")
         if d.variant == STREAM:
-            bits = 32 if self.f32 else 64
-            ity = "int" if self.f32 else "__attribute__((ext_vector_type(2))) int"
             w("// Buffer access: per-lane byte offset `off` (range-checked: lanes given an")
             w("// offset past the buffer read 0 and their stores are dropped) plus a")
             w("// wave-uniform non-negative row/plane offset `soff` in an SGPR.")
-            w(f"__device__ __forceinline__ {T} tcsg_ld(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff) {{")
-            w(f"  return __builtin_bit_cast({T}, __builtin_amdgcn_raw_buffer_load_b{bits}(r, off, soff, 0));")
-            w("}")
-            aux = 2 if d.nt_store else 0
-            w(f"__device__ __forceinline__ void tcsg_st(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff, {T} v) {{")
-            w(f"  __builtin_amdgcn_raw_buffer_store_b{bits}(__builtin_bit_cast({ity}, v), r, off, soff, {aux});")
-            w("}")
+            if cpu:
+                w("inline tcsg_emu_rsrc tcsg_rsrc(const void* p, uint32_t n) { return tcsg_make_rsrc(p, n); }")
+                w(f"inline TV tcsg_ld(const tcsg_emu_rsrc& r, const u32& off, const u32& soff) {{ return tcsg_emu_ld<{T}>(r, off, soff); }}")
+                w("inline void tcsg_st(const tcsg_emu_rsrc& r, const u32& off, const u32& soff, const TV& v) { tcsg_emu_st(r, off, soff, v); }")
+            else:
+                bits = 32 if self.f32 else 64
+                ity = "int" if self.f32 else "__attribute__((ext_vector_type(2))) int"
+                w("__device__ __forceinline__ __amdgpu_buffer_rsrc_t tcsg_rsrc(const void* p, uint32_t n) {")
+                w("  return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)n, 0x00020000);")
+                w("}")
+                w(f"__device__ __forceinline__ {T} tcsg_ld(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff) {{")
+                w(f"  return __builtin_bit_cast({T}, __builtin_amdgcn_raw_buffer_load_b{bits}(r, off, soff, 0));")
+                w("}")
+                aux = 2 if d.nt_store else 0
+                w(f"__device__ __forceinline__ void tcsg_st(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff, {T} v) {{")
+                w(f"  __builtin_amdgcn_raw_buffer_store_b{bits}(__builtin_bit_cast({ity}, v), r, off, soff, {aux});")
+                w("}")
         w("")
+
+    def emulation_runner(self):
+        """Host loop that runs every wave of a launch through the emulated kernel."""
+        w = self.w
+        w("tcsg_emu_dim3u blockIdx, blockDim, gridDim;")
+        w("tcsg_emu_dim3l threadIdx;")
+        w('extern "C" { unsigned long tcsg_emu_bad_access = 0; }')
+        w('extern "C" int tcsg_emulate(const tcsg_kargs* a, uint32_t gx, uint32_t gy, uint32_t gz,')
+        w("                            uint32_t bx, uint32_t by, uint32_t bz) {")
+        w("  const uint32_t nt = bx * by * bz;")
+        w("  if (nt == 0 || nt % 64) return -1;  // whole waves only")
+        w("  gridDim = {gx, gy, gz};")
+        w("  blockDim = {bx, by, bz};")
+        w("  for (uint32_t k = 0; k < gz; ++k) for (uint32_t j = 0; j < gy; ++j) for (uint32_t i = 0; i < gx; ++i) {")
+        w("    blockIdx = {i, j, k};")
+        w("    for (uint32_t w0 = 0; w0 < nt; w0 += 64) {")
+        w("      for (int l = 0; l < 64; ++l) {")
+        w("        const uint32_t lid = w0 + l;")
+        w("        threadIdx.x.v[l] = lid % bx; threadIdx.y.v[l] = (lid / bx) % by; threadIdx.z.v[l] = lid / (bx * by);")
+        w("      }")
+        w("      tcsg_update(*a);")
+        w("    }")
+        w("  }")
+        w("  return 0;")
+        w("}")
 
     # ------------------------------------------------------------ body parts
     def block_coords(self):
@@ -226,36 +274,36 @@ class _Emitter:
         if self.d.xcd_remap:
             w("  // XCD-aware remap: hardware hands consecutive workgroups to the 8 XCDs")
             w("  // round-robin; give each XCD a contiguous range of tiles instead.")
-            w("  const uint32_t nbx = gridDim.x, nby = gridDim.y, nbz = gridDim.z;")
-            w("  const uint32_t nblk = nbx * nby * nbz;")
-            w("  uint32_t lin = blockIdx.x + nbx * (blockIdx.y + nby * blockIdx.z);")
+            w("  const u32 nbx = gridDim.x, nby = gridDim.y, nbz = gridDim.z;")
+            w("  const u32 nblk = nbx * nby * nbz;")
+            w("  u32 lin = blockIdx.x + nbx * (blockIdx.y + nby * blockIdx.z);")
             w("  {")
-            w("    const uint32_t q = nblk >> 3, rm = nblk & 7u, xcd = lin & 7u, k = lin >> 3;")
-            w("    lin = xcd < rm ? xcd * (q + 1u) + k : rm * (q + 1u) + (xcd - rm) * q + k;")
+            w("    const u32 q = nblk >> 3, rm = nblk & 7u, xcd = lin & 7u, k = lin >> 3;")
+            w("    lin = tcsg_sel(xcd < rm, xcd * (q + 1u) + k, rm * (q + 1u) + (xcd - rm) * q + k);")
             w("  }")
             if self.d.zfirst:
-                w("  const uint32_t bzi = lin % nbz, byi = (lin / nbz) % nby, bxi = lin / (nbz * nby);")
+                w("  const u32 bzi = lin % nbz, byi = (lin / nbz) % nby, bxi = lin / (nbz * nby);")
             else:
-                w("  const uint32_t bxi = lin % nbx, byi = (lin / nbx) % nby, bzi = lin / (nbx * nby);")
+                w("  const u32 bxi = lin % nbx, byi = (lin / nbx) % nby, bzi = lin / (nbx * nby);")
         else:
-            w("  const uint32_t bxi = blockIdx.x, byi = blockIdx.y, bzi = blockIdx.z;")
-        w("  const uint32_t gx = bxi * blockDim.x + threadIdx.x;")
-        w("  const uint32_t gy = byi * blockDim.y + threadIdx.y;")
-        w("  const uint32_t gz = bzi * blockDim.z + threadIdx.z;")
+            w("  const u32 bxi = blockIdx.x, byi = blockIdx.y, bzi = blockIdx.z;")
+        w("  const u32 gx = bxi * blockDim.x + threadIdx.x;")
+        w("  const u32 gy = byi * blockDim.y + threadIdx.y;")
+        w("  const u32 gz = bzi * blockDim.z + threadIdx.z;")
 
     def thread_counts(self):
         d = self.d
         R, LO = self.R, d.left_offset
         w = self.w
-        w("  const uint32_t xdim = a.xdim, ydim = a.ydim;")
-        w(f"  const uint32_t ex = xdim - a.right_offset;  // end of the x extent in use")
-        w(f"  const uint32_t thx = ex >= {2 * R + LO} ? (ex - {2 * R + LO}) / {self.XC}u : 0u;")
-        w(f"  const uint32_t thy = ydim >= {2 * R} ? (ydim - {2 * R}) / {self.YC}u : 0u;")
+        w("  const u32 xdim = a.xdim, ydim = a.ydim;")
+        w(f"  const u32 ex = xdim - a.right_offset;  // end of the x extent in use")
+        w(f"  const u32 thx = tcsg_sel(ex >= {2 * R + LO}u, (ex - {2 * R + LO}u) / {self.XC}u, 0u);")
+        w(f"  const u32 thy = tcsg_sel(ydim >= {2 * R}u, (ydim - {2 * R}u) / {self.YC}u, 0u);")
         if self.three:
-            w("  const uint32_t zdim = a.zdim;")
-            w(f"  const uint32_t thz = zdim >= {2 * R} ? (zdim - {2 * R}) / {self.ZC}u : 0u;")
+            w("  const u32 zdim = a.zdim;")
+            w(f"  const u32 thz = tcsg_sel(zdim >= {2 * R}u, (zdim - {2 * R}u) / {self.ZC}u, 0u);")
         else:
-            w("  const uint32_t thz = 1u;")
+            w("  const u32 thz = 1u;")
 
     # ---------------------------------------------------------------- PLAIN
     def body_plain(self):
@@ -303,7 +351,7 @@ class _Emitter:
         T = self.T
         self.w(f"  const {T}* __restrict__ mk = a.mask;")
         for m in range(self.d.n):
-            self.w(f"  const {T} w{m} = mk[{m}];")
+            self.w(f"  const TV w{m} = mk[{m}];")
 
     # --------------------------------------------------------------- STREAM
     def halo_lanes(self) -> int:
@@ -327,44 +375,57 @@ class _Emitter:
         w("  // inrow: the thread's row/plane is inside the interior; such lanes load (and")
         w("  // can feed x-neighbours to other lanes) even past the interior's x end, where")
         w("  // they read the halo.  valid: the thread owns interior points and stores them.")
-        w("  const bool inrow = gy < thy && gz < thz;")
-        w("  const bool valid = inrow && gx < thx;")
-        w(f"  const uint32_t x0 = gx * {XC}u + {R + LO}u;")
-        w(f"  const uint32_t y0 = gy * {self.YC}u;            // first row of the footprint")
-        w(f"  const uint32_t z0 = gz * {self.ZC}u;            // first plane of the footprint" if self.three
-          else "  const uint32_t z0 = 0u;")
-        w(f"  const uint32_t sx = xdim * {es}u;            // row pitch in bytes")
-        w(f"  const uint32_t sxy = xdim * ydim * {es}u;     // plane pitch in bytes")
-        w(f"  const uint32_t real = (z0 * ydim + y0) * xdim * {es}u + (x0 - {XL}u) * {es}u;")
+        w("  const lbool inrow = gy < thy && gz < thz;")
+        w("  const lbool valid = inrow && gx < thx;")
+        w(f"  const u32 x0 = gx * {XC}u + {R + LO}u;")
+        w(f"  const u32 y0 = gy * {self.YC}u;            // first row of the footprint")
+        w(f"  const u32 z0 = gz * {self.ZC}u;            // first plane of the footprint" if self.three
+          else "  const u32 z0 = 0u;")
+        w(f"  const u32 sx = xdim * {es}u;            // row pitch in bytes")
+        w(f"  const u32 sxy = xdim * ydim * {es}u;     // plane pitch in bytes")
+        w(f"  const u32 real = (z0 * ydim + y0) * xdim * {es}u + (x0 - {XL}u) * {es}u;")
         w(f"  // offsets past the buffer: loads read 0, stores are dropped by the range check")
-        w(f"  const uint32_t base = inrow ? real : {OOB_BASE:#x}u;")
-        w(f"  const uint32_t sbase = valid ? real : {OOB_BASE:#x}u;")
-        w("  const __amdgpu_buffer_rsrc_t rin = __builtin_amdgcn_make_buffer_rsrc((void*)a.in, (short)0, (int)a.in_bytes, 0x00020000);")
-        w("  const __amdgpu_buffer_rsrc_t rout = __builtin_amdgcn_make_buffer_rsrc((void*)a.out, (short)0, (int)a.out_bytes, 0x00020000);")
+        if dpp:
+            w("  // Lanes past the interior's x end still load the columns of their rows that lie")
+            w("  // inside the row (valid lanes take halo columns from them through DPP); every")
+            w("  // other access of a non-valid lane gets an out-of-range per-lane offset, so the")
+            w("  // row/plane offset carried in the SGPR never leads past the buffer.")
+            w(f"  const u32 base = tcsg_sel(inrow, real, {OOB_BASE:#x}u);")
+        else:
+            w(f"  const u32 base = tcsg_sel(valid, real, {OOB_BASE:#x}u);")
+        w(f"  const u32 sbase = tcsg_sel(valid, real, {OOB_BASE:#x}u);")
+        w("  const auto rin = tcsg_rsrc(a.in, a.in_bytes);")
+        w("  const auto rout = tcsg_rsrc(a.out, a.out_bytes);")
         self.weights()
         if dpp:
-            w("  const uint32_t lid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);")
-            w("  const uint32_t lane = lid & 63u;")
+            w("  const u32 lid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);")
+            w("  const u32 lane = lid & 63u;")
             if wave_aligned:
                 w("  // Block rows are whole waves (the runtime only launches block x extents")
                 w(f"  // that are multiples of 64): the {H} edge lanes of each wave load the")
                 w("  // columns beyond the wave, which enter the lane-shift chains as the DPP")
                 w("  // `old` value of lanes that have no source lane.")
-                w(f"  const bool edgeL = lane < {H}u, edgeR = lane + {H}u > 63u;")
+                w(f"  const lbool edgeL = lane < {H}u, edgeR = lane + {H}u > 63u;")
             else:
                 w("  // Lane segments: runs of lanes in one block row and one wave.  tL/tR are the")
                 w("  // lane's distances to its segment's ends; the runtime only launches block")
                 w(f"  // shapes whose segments are at least H = {H} lanes long.")
-                w("  const uint32_t tx = threadIdx.x;")
-                w("  const uint32_t tL = min(tx, lane);")
-                w("  const uint32_t tR = min(blockDim.x - 1u - tx, 63u - lane);")
-                w(f"  const bool edgeL = tL < {H}u, edgeR = tR < {H}u;")
+                w("  const u32 tx = threadIdx.x;")
+                w("  const u32 tL = tcsg_min(tx, lane);")
+                w("  const u32 tR = tcsg_min(blockDim.x - 1u - tx, 63u - lane);")
+                w(f"  const lbool edgeL = tL < {H}u, edgeR = tR < {H}u;")
         if dpp:
-            w("  // Extra columns of the edge lanes; the left ones are addressed from their own")
-            w("  // offsets so that no negative immediate is folded into a load.")
+            w("  // Per-column offsets: own columns, and the extra columns of the segment-edge")
+            w("  // lanes (left ones addressed from their own offsets so that no negative")
+            w("  // immediate is folded into a load); a column at or past the row end is out of range.")
             for ip in range(XC):
-                w(f"  const uint32_t eoffL{ip} = edgeL ? base - {(H * XC - ip) * es}u : {OOB_BASE:#x}u;")
-            w(f"  const uint32_t eoffR = edgeR ? base + {H * XC * es}u : {OOB_BASE:#x}u;")
+                w(f"  const u32 coff{ip} = tcsg_sel(inrow && x0 + {ip}u < xdim, real + {ip * es}u, {OOB_BASE:#x}u);")
+            for ip in range(XC):
+                k = H * XC - ip
+                w(f"  const u32 eoffL{ip} = tcsg_sel(edgeL && inrow && x0 - {k}u < xdim, real - {k * es}u, {OOB_BASE:#x}u);")
+            for ip in range(XC):
+                k = H * XC + ip
+                w(f"  const u32 eoffR{ip} = tcsg_sel(edgeR && inrow && x0 + {k}u < xdim, real + {k * es}u, {OOB_BASE:#x}u);")
         pts, contrib = self.footprint()
         last = {}
         for idx, p in enumerate(pts):
@@ -418,11 +479,11 @@ class _Emitter:
                     if sp[0] in ("EL", "ER"):
                         _, ip, py2, pz2 = sp
                         side = sp[0][1]
-                        off = f"eoffL{ip}" if side == "L" else f"eoffR + {ip * es}u"
-                        out.append((f"e{side}0_{ip}_{_n(py2)}_{_n(pz2)}", off, soff(py2, pz2)))
+                        out.append((f"e{side}0_{ip}_{_n(py2)}_{_n(pz2)}", f"eoff{side}{ip}", soff(py2, pz2)))
                     else:
                         qx, qy, qz = sp
-                        out.append((vname(qx, qy, qz), f"base + {(qx + XL) * es}u", soff(qy, qz)))
+                        off = f"coff{qx}" if dpp else f"base + {(qx + XL) * es}u"
+                        out.append((vname(qx, qy, qz), off, soff(qy, qz)))
             return out
 
         declared = set()
@@ -432,7 +493,7 @@ class _Emitter:
                 if nm in declared:
                     continue
                 declared.add(nm)
-                w(f"  const {T} {nm} = tcsg_ld(rin, {off}, {so});")
+                w(f"  const TV {nm} = tcsg_ld(rin, {off}, {so});")
 
         D = d.prefetch
         for i in range(min(D, len(order))):
@@ -456,25 +517,25 @@ class _Emitter:
                     for j in range(1, H):
                         if gname(j) not in declared:
                             declared.add(gname(j))
-                            w(f"  const {T} {gname(j)} = {e_fn}({gname(j - 1)});")
+                            w(f"  const TV {gname(j)} = {e_fn}({gname(j - 1)});")
                     if wave_aligned:
                         prev = vname(ip, py, pz)
                         for hh in range(1, h + 1):
                             sn = f"m{side}{hh}_{ip}_{_n(py)}_{_n(pz)}"
                             if sn not in declared:
                                 declared.add(sn)
-                                w(f"  const {T} {sn} = {main_fn}_or({gname(H - hh)}, {prev});")
+                                w(f"  const TV {sn} = {main_fn}_or({gname(H - hh)}, {prev});")
                             prev = sn
-                        w(f"  const {T} {nm} = {prev};")
+                        w(f"  const TV {nm} = {prev};")
                     else:
                         prev = vname(ip, py, pz)
                         for hh in range(1, h + 1):
                             sn = f"s{side}{hh}_{ip}_{_n(py)}_{_n(pz)}"
                             if sn not in declared:
                                 declared.add(sn)
-                                w(f"  const {T} {sn} = {main_fn}({prev});")
+                                w(f"  const TV {sn} = {main_fn}({prev});")
                             prev = sn
-                        w(f"  const {T} {nm} = t{side} >= {h}u ? {prev} : {gname(H - h)};")
+                        w(f"  const TV {nm} = tcsg_sel(t{side} >= {h}u, {prev}, {gname(H - h)});")
                     declared.add(nm)
             for idx, p in members[k]:
                 v = vname(*p)
@@ -490,7 +551,7 @@ class _Emitter:
         d = self.d
         self.preamble()
         attr = ""
-        if d.variant == STREAM and d.waves_per_eu:
+        if d.variant == STREAM and d.waves_per_eu and self.target != "cpu":
             attr = f" __attribute__((amdgpu_waves_per_eu({d.waves_per_eu})))"
         self.w(f'extern "C" __global__ void __launch_bounds__({d.max_threads}){attr} tcsg_update(const tcsg_kargs a)')
         self.w("{")
@@ -499,9 +560,30 @@ class _Emitter:
         else:
             self.body_stream()
         self.w("}")
+        if self.target == "cpu":
+            self.emulation_runner()
         return "\n".join(self.L) + "\n"
 
 
-def emit_source(desc: StencilDesc) -> str:
-    """HIP source text of the kernel for ``desc``."""
-    return _Emitter(desc).source()
+# Straight-line kernels grow with outputs-per-thread x stencil size; past this
+# many accumulate statements hipcc takes minutes and the code no longer fits
+# the instruction cache, so the generator refuses the parameter set.
+MAX_TERMS_PER_THREAD = 40000
+
+
+def terms_per_thread(desc: StencilDesc) -> int:
+    return desc.xc * desc.yc * desc.zc * desc.n
+
+
+def emit_source(desc: StencilDesc, target: str = "gfx950") -> str:
+    """Source text of the kernel for ``desc``: HIP for gfx950, or (target
+    "cpu", STREAM variant only) the same kernel body over the wave-level CPU
+    emulation in cpu_emu.h, used to test the emitted code on the host."""
+    if terms_per_thread(desc) > MAX_TERMS_PER_THREAD:
+        raise ValueError(f"coarsening {desc.coarsening} x stencil size {desc.n} = {terms_per_thread(desc)} "
+                         f"accumulate statements per thread exceeds {MAX_TERMS_PER_THREAD}")
+    if target == "cpu" and desc.variant != STREAM:
+        raise ValueError("CPU emulation covers the STREAM variant")
+    if target not in ("gfx950", "cpu"):
+        raise ValueError(f"unknown target {target}")
+    return _Emitter(desc, target).source()

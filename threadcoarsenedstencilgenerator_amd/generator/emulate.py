@@ -1,0 +1,65 @@
+"""Run generated STREAM kernels on the host through the wave-level CPU
+emulation (cpu_emu.h): the same kernel text as the gfx950 build, with every
+lane of a wave, its DPP moves and its range-checked buffer accesses
+emulated.  Used by the CPU test suite to check the emitted code against the
+oracle without a GPU; the GPU suite checks the compiled code objects."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+
+from .. import spec
+from ..spec import F32C
+from .emit import emit_source
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+EMU_DIR = os.path.join(os.path.dirname(HERE), "kernels", "emu")
+
+
+class _KArgs(C.Structure):
+    _fields_ = [("in_", C.c_void_p), ("out", C.c_void_p), ("mask", C.c_void_p),
+                ("xdim", C.c_uint32), ("ydim", C.c_uint32), ("zdim", C.c_uint32), ("right_offset", C.c_uint32),
+                ("in_bytes", C.c_uint32), ("out_bytes", C.c_uint32), ("pad0", C.c_uint32), ("pad1", C.c_uint32)]
+
+
+def build(desc: spec.StencilDesc) -> str:
+    os.makedirs(EMU_DIR, exist_ok=True)
+    so = os.path.join(EMU_DIR, desc.key() + ".so")
+    src_text = emit_source(desc, "cpu")
+    stamp = so + ".src"
+    if os.path.exists(so) and os.path.exists(stamp) and open(stamp).read() == src_text:
+        return so
+    with tempfile.TemporaryDirectory(prefix="tcsg_emu_") as td:
+        src = os.path.join(td, "k.cpp")
+        open(src, "w").write(src_text)
+        tmp = so + f".{os.getpid()}.tmp"
+        r = subprocess.run(["g++", "-O1", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", f"-I{HERE}",
+                            src, "-o", tmp], capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"emulation build failed for {desc.key()}:\n{r.stderr[-4000:]}")
+        os.replace(tmp, so)
+    open(stamp, "w").write(src_text)
+    return so
+
+
+def run(desc: spec.StencilDesc, geom, mask, a: np.ndarray, b: np.ndarray, steps: int, block, blocks):
+    """`steps` ping-pong steps of the emulated kernel on host arrays a/b
+    (geometry as from spec.geometry).  Returns (final array, bad accesses)."""
+    L = C.CDLL(build(desc))
+    L.tcsg_emulate.argtypes = [C.c_void_p] + [C.c_uint32] * 6
+    dt = np.float32 if desc.dtype == F32C else np.float64
+    m = np.ascontiguousarray(mask if mask is not None else np.zeros(1), dtype=dt)
+    bufs = [a, b]
+    nbytes = a.nbytes
+    for s in range(steps):
+        k = _KArgs(bufs[s & 1].ctypes.data, bufs[(s + 1) & 1].ctypes.data, m.ctypes.data,
+                   geom.xdim, geom.ydim, geom.zdim, geom.ro, nbytes, nbytes, 0, 0)
+        rc = L.tcsg_emulate(C.byref(k), *blocks, *block)
+        if rc != 0:
+            raise ValueError("block shape must be whole waves")
+    bad = C.c_ulong.in_dll(L, "tcsg_emu_bad_access").value
+    return bufs[steps & 1], bad

@@ -101,6 +101,16 @@ def cfg2_tuned():
     return out
 
 
+def cfg2_f32c_candidates():
+    """FP32-storage compensated kernels for the 512^3 workload."""
+    out = []
+    for (x, y, z) in ((2, 4, 128), (4, 2, 128), (2, 8, 64), (4, 2, 64), (4, 3, 64), (2, 6, 64)):
+        for g in (1, 2, 5, 8, 13, 25):
+            out.append(cfg2_desc(x, y, z, shape="dppw", max_threads=256, dtype=F32C).replace(prefetch=0,
+                                                                                          comp_group=g))
+    return out
+
+
 def cfg3_desc(xc=1, yc=1, orientation=ROW, dtype=F64, shape="plain", max_threads=1024):
     """2D Box Variable r=8, alignment Read at 16 points."""
     return StencilDesc(dimension=2, neighbourhood=BOX, orientation=orientation, coeff=VARIABLE, range=8, xc=xc,
@@ -117,7 +127,7 @@ def cfg4_desc(dtype=F64, shape="dpp"):
 def default_build_set():
     descs = [d for d, _ in parity_cases()] + [d for d, _ in f32c_cases()]
     descs += [cfg1_desc("plain"), cfg1_desc("stream"), cfg1_desc("dpp")]
-    descs += cfg2_sweep() + cfg2_tuned()
+    descs += cfg2_sweep() + cfg2_tuned() + cfg2_f32c_candidates()
     descs += [cfg4_desc(F64, "plain"), cfg4_desc(F64, "dpp")]
     from . import tuned
     descs += tuned.build_set()

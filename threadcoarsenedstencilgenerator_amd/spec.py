@@ -126,7 +126,8 @@ class StencilDesc:
         return (f"{self.dimension}d_{nb}_r{self.range}_{_inv(COEFF_NAMES)[self.coeff].lower()}"
                 f"_{_inv(ORIENT_NAMES)[self.orientation].lower()}_c{self.xc}x{self.yc}x{self.zc}"
                 f"_{_inv(ALIGN_NAMES)[self.alignment].lower()}{self.cl_size}"
-                f"_{_inv(DTYPE_NAMES)[self.dtype]}_{_inv(VARIANT_NAMES)[self.variant]}_{h}")
+                f"_{_inv(DTYPE_NAMES)[self.dtype]}_{_inv(VARIANT_NAMES)[self.variant]}"
+                f"{'' if self.variant == PLAIN else '_' + self.xext}_{h}")
 
     def params_blob(self):
         """Integers baked into the code object and checked by the runtime

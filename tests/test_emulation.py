@@ -54,7 +54,7 @@ def test_emulated_stream_kernel_matches_oracle(case):
     for block in blocks:
         nblk = tuple((geo.threads[i] + block[i] - 1) // block[i] for i in range(3))
         got, bad = emulate.run(d, geo, mask.astype(dt), a.copy(), a.copy(), 2, block, nblk)
-        assert bad == 0, f"{bad} accesses past the buffer that the range check would not catch"
+        assert bad == 0, f"{bad} accesses whose full offset does not fit in 32 bits"
         ui = np.uint32 if dt == np.float32 else np.uint64
         diff = np.flatnonzero(got.view(ui) != ref.view(ui))
         assert diff.size == 0, f"block {block}: {diff.size} points differ, first {diff[:5]}"

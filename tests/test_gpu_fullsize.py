@@ -72,7 +72,9 @@ def test_f32c_error_vs_f64_after_20_steps():
     for i in range(2):
         rt.convert_f64_to_f32(g64.ptr(0), g32.ptr(i), g.n_points)
     f64 = k64.run(g, b64, rt.grid_blocks(g, b64), g64.ptr(0), g64.ptr(1), 20)
-    f32 = k32.run(g, b32, rt.grid_blocks(g, b32), g32.ptr(0), g32.ptr(1), 20)
+    gf = k32.geometry(GRID)  # same padded layout, its own thread counts
+    assert (gf.size.x, gf.size.y, gf.size.z) == (g.size.x, g.size.y, g.size.z)
+    f32 = k32.run(gf, b32, rt.grid_blocks(gf, b32), g32.ptr(0), g32.ptr(1), 20)
     torch.cuda.synchronize()
     mx, mb, s2, b2, _ = rt.error_f32_vs_f64(g, g32.ptr(f32), g64.ptr(f64), 1e-3)
     print(f"f32c vs f64 after 20 steps: normwise max {mx / mb:.3e}, rms {np.sqrt(s2 / b2):.3e}", flush=True)

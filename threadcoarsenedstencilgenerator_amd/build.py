@@ -40,6 +40,17 @@ def build_runtime() -> str:
     return LIB
 
 
+PROBE_CO = os.path.join(KDIR, "buffer_probe.co")
+
+
+def build_probe() -> str:
+    """Code object of the buffer range-check probe (csrc/buffer_probe.hip)."""
+    os.makedirs(KDIR, exist_ok=True)
+    _run([HIPCC, *HIP_FLAGS, "--genco", "--no-gpu-bundle-output", os.path.join(PKG, "csrc", "buffer_probe.hip"),
+          "-o", PROBE_CO])
+    return PROBE_CO
+
+
 def kernel_path(desc: spec.StencilDesc) -> str:
     return os.path.join(KDIR, desc.key() + ".co")
 

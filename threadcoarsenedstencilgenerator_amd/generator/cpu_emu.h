@@ -5,12 +5,12 @@
 // One call of the kernel function executes one wave: every per-lane value is
 // a Lane<T> holding the 64 lanes, so the cross-lane DPP moves are emulated
 // exactly (wave_shr:1 / wave_shl:1, with bound_ctrl zeroing or an `old`
-// operand).  Buffer loads and stores are range-checked like raw buffer
-// resources (stride 0): a lane whose per-lane offset is at or past the record
-// count reads 0 and does not store.  An access whose per-lane offset is in
-// range but whose full offset (plus the wave-uniform SGPR offset) is not is
-// counted in tcsg_emu_bad_access: on hardware it could reach memory past the
-// buffer, so tests require the count to stay 0.
+// operand).  Buffer loads and stores follow the raw-buffer range check
+// measured on gfx950 (stride 0): each element is in range iff its full offset
+// (per-lane + wave-uniform SGPR offset) plus its size is within the record
+// count; out-of-range elements read 0 and are not stored.  A full offset that
+// does not fit in 32 bits is counted in tcsg_emu_bad_access (its hardware
+// meaning would depend on wrap-around); tests require the count to stay 0.
 #pragma once
 #include <cmath>
 #include <cstdint>
@@ -165,9 +165,9 @@ Lane<T> tcsg_emu_ld(const tcsg_emu_rsrc &r, const u32 &off, const u32 &soff)
 {
     Lane<T> out(0);
     for (int i = 0; i < tcsg_emu::W; ++i) {
-        const uint64_t o = off.v[i], full = o + soff.v[i];
-        if (o >= r.records) continue;  // range check on the per-lane offset: reads 0
-        if (full + sizeof(T) > r.records) { ++tcsg_emu_bad_access; continue; }
+        const uint64_t full = (uint64_t)off.v[i] + soff.v[i];
+        if (full >> 32) { ++tcsg_emu_bad_access; continue; }
+        if (full + sizeof(T) > r.records) continue;  // out of range: reads 0
         std::memcpy(&out.v[i], r.base + full, sizeof(T));
     }
     return out;
@@ -177,9 +177,9 @@ template <class T>
 void tcsg_emu_st(const tcsg_emu_rsrc &r, const u32 &off, const u32 &soff, const Lane<T> &v)
 {
     for (int i = 0; i < tcsg_emu::W; ++i) {
-        const uint64_t o = off.v[i], full = o + soff.v[i];
-        if (o >= r.records) continue;  // dropped
-        if (full + sizeof(T) > r.records) { ++tcsg_emu_bad_access; continue; }
+        const uint64_t full = (uint64_t)off.v[i] + soff.v[i];
+        if (full >> 32) { ++tcsg_emu_bad_access; continue; }
+        if (full + sizeof(T) > r.records) continue;  // out of range: dropped
         std::memcpy(r.base + full, &v.v[i], sizeof(T));
     }
 }

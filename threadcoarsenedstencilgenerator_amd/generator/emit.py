@@ -386,10 +386,10 @@ class _Emitter:
         w(f"  const u32 real = (z0 * ydim + y0) * xdim * {es}u + (x0 - {XL}u) * {es}u;")
         w(f"  // offsets past the buffer: loads read 0, stores are dropped by the range check")
         if dpp:
-            w("  // Lanes past the interior's x end still load the columns of their rows that lie")
-            w("  // inside the row (valid lanes take halo columns from them through DPP); every")
-            w("  // other access of a non-valid lane gets an out-of-range per-lane offset, so the")
-            w("  // row/plane offset carried in the SGPR never leads past the buffer.")
+            w("  // Lanes past the interior's x end still load (valid lanes take halo columns from")
+            w("  // them through DPP).  Buffer range checks are per dword on the full offset")
+            w("  // (per-lane + SGPR + immediate), so anything such a lane reads past the buffer's")
+            w("  // end returns 0 and the in-range part of a straddling access is exact.")
             w(f"  const u32 base = tcsg_sel(inrow, real, {OOB_BASE:#x}u);")
         else:
             w(f"  const u32 base = tcsg_sel(valid, real, {OOB_BASE:#x}u);")
@@ -415,17 +415,13 @@ class _Emitter:
                 w("  const u32 tR = tcsg_min(blockDim.x - 1u - tx, 63u - lane);")
                 w(f"  const lbool edgeL = tL < {H}u, edgeR = tR < {H}u;")
         if dpp:
-            w("  // Per-column offsets: own columns, and the extra columns of the segment-edge")
-            w("  // lanes (left ones addressed from their own offsets so that no negative")
-            w("  // immediate is folded into a load); a column at or past the row end is out of range.")
-            for ip in range(XC):
-                w(f"  const u32 coff{ip} = tcsg_sel(inrow && x0 + {ip}u < xdim, real + {ip * es}u, {OOB_BASE:#x}u);")
+            w("  // Extra columns of the segment-edge lanes.  Left ones are addressed from their")
+            w("  // own offsets, only where the column index stays non-negative, so no per-lane")
+            w("  // offset wraps below zero.")
             for ip in range(XC):
                 k = H * XC - ip
-                w(f"  const u32 eoffL{ip} = tcsg_sel(edgeL && inrow && x0 - {k}u < xdim, real - {k * es}u, {OOB_BASE:#x}u);")
-            for ip in range(XC):
-                k = H * XC + ip
-                w(f"  const u32 eoffR{ip} = tcsg_sel(edgeR && inrow && x0 + {k}u < xdim, real + {k * es}u, {OOB_BASE:#x}u);")
+                w(f"  const u32 eoffL{ip} = tcsg_sel(edgeL && x0 >= {k}u, base - {k * es}u, {OOB_BASE:#x}u);")
+            w(f"  const u32 eoffR = tcsg_sel(edgeR, base + {H * XC * es}u, {OOB_BASE:#x}u);")
         pts, contrib = self.footprint()
         last = {}
         for idx, p in enumerate(pts):
@@ -479,11 +475,11 @@ class _Emitter:
                     if sp[0] in ("EL", "ER"):
                         _, ip, py2, pz2 = sp
                         side = sp[0][1]
-                        out.append((f"e{side}0_{ip}_{_n(py2)}_{_n(pz2)}", f"eoff{side}{ip}", soff(py2, pz2)))
+                        off = f"eoffL{ip}" if side == "L" else f"eoffR + {ip * es}u"
+                        out.append((f"e{side}0_{ip}_{_n(py2)}_{_n(pz2)}", off, soff(py2, pz2)))
                     else:
                         qx, qy, qz = sp
-                        off = f"coff{qx}" if dpp else f"base + {(qx + XL) * es}u"
-                        out.append((vname(qx, qy, qz), off, soff(qy, qz)))
+                        out.append((vname(qx, qy, qz), f"base + {(qx + XL) * es}u", soff(qy, qz)))
             return out
 
         declared = set()

@@ -125,10 +125,13 @@ def cfg4_desc(dtype=F64, shape="dpp"):
 
 
 def default_build_set():
+    """Kernels the tests, smoke() and bench.py load.  Tuning candidate sets
+    (cfg2_sweep, cfg2_tuned, cfg2_f32c_candidates, ...) are compiled on
+    demand by the sweeps and cached in the same directory."""
     descs = [d for d, _ in parity_cases()] + [d for d, _ in f32c_cases()]
-    descs += [cfg1_desc("plain"), cfg1_desc("stream"), cfg1_desc("dpp")]
-    descs += cfg2_sweep() + cfg2_tuned() + cfg2_f32c_candidates()
-    descs += [cfg4_desc(F64, "plain"), cfg4_desc(F64, "dpp")]
+    descs += [cfg1_desc("plain"), cfg1_desc("stream"), cfg1_desc("dppw")]
+    descs += [cfg2_desc(1, 1, 1), cfg2_desc(2, 4, 8)]            # reference-shaped kernels
+    descs += [cfg4_desc(F64, "plain"), cfg4_desc(F64, "dppw")]
     from . import tuned
     descs += tuned.build_set()
     return list(dict.fromkeys(descs))

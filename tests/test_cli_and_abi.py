@@ -1,0 +1,91 @@
+"""CPU checks of the drop-in surfaces: the generator command lines (the
+reference's argv and error behaviour), the C ABI exports of libtcsg.so, the
+launchUpdateGrid.h compatibility unit, and geometry validation through the C
+ABI (no GPU call)."""
+import ctypes as C
+import os
+import re
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "threadcoarsenedstencilgenerator_amd")
+
+
+def _gen(script, *args):
+    return subprocess.run([sys.executable, os.path.join(PKG, script), *args], capture_output=True, text=True)
+
+
+def test_generator_argv_like_reference():
+    r = _gen("generate2DStencil.py", "Box", "Row", "Jacobi", "1", "1", "1")
+    assert r.returncode == 0 and "tcsg_update" in r.stdout
+    r = _gen("generate3DStencil.py", "Star", "Column", "Variable", "4", "1", "2", "8", "Write", "16")
+    assert r.returncode == 0 and "tcsg_params[24] = {1413698375, 1, 3, 1, 1, 1, 4, 1, 2, 8, 2, 16," in r.stdout
+    # wrong argument counts / values: usage on stderr, status 1 (hip/generate2DStencil_hip.py:28-60)
+    for args in (("Box", "Row", "Jacobi", "1", "1"), ("Boxx", "Row", "Jacobi", "1", "1", "1"),
+                 ("Box", "Row", "Jacobi", "0", "1", "1"), ("Box", "Row", "Jacobi", "1", "0", "1"),
+                 ("Box", "Row", "Jacobi", "1", "1", "1", "Write", "0"),
+                 ("Box", "Row", "Jacobi", "1", "1", "1", "Left", "16")):
+        r = _gen("generate2DStencil.py", *args)
+        assert r.returncode == 1 and "usage" in r.stderr
+    r = _gen("generate3DStencil.py", "Box", "Row", "Jacobi", "1", "1", "1")
+    assert r.returncode == 1
+
+
+def test_launch_interface_generator():
+    r = _gen("generateLaunchUpdateGrid.py", "Star", "Variable", "4", "3D", "Write")
+    assert r.returncode == 0 and '#define TCSG_EXPECT_ALIGNMENT "Write"' in r.stdout
+    assert _gen("generateLaunchUpdateGrid.py", "Star", "Variable", "4", "4D").returncode == 1
+
+
+def test_c_abi_exports_every_declared_symbol():
+    from threadcoarsenedstencilgenerator_amd.build import LIB
+    header = open(os.path.join(ROOT, "include", "tcsg.h")).read()
+    declared = set(re.findall(r"\b(tcsg_[a-z0-9_]+)\s*\(", header))
+    assert len(declared) >= 18
+    lib = C.CDLL(LIB)
+    for name in declared:
+        assert hasattr(lib, name), name
+
+
+def test_launchupdategrid_header_compiles_and_links():
+    """A harness written against the reference's four launchUpdateGrid.h
+    signatures builds against the compatibility library."""
+    from threadcoarsenedstencilgenerator_amd.build import PKG as P, SHIM_LIB
+    src = r'''
+#include "launchUpdateGrid.h"
+int main() {
+  std::vector<double> (*a)(std::string, std::string, unsigned, unsigned, std::string, double, dim3, dim3, dim3,
+                           double*, double*, unsigned) = benchmarkUpdateGrid;
+  void (*b)(dim3, dim3, dim3, double*, unsigned) = launchInit;
+  int (*c)() = maxPotentialBlockSize;
+  void (*d)(double*, unsigned) = copyMask;
+  return (a && b && c && d) ? 0 : 1;
+}'''
+    import tempfile
+    with tempfile.TemporaryDirectory() as td:
+        f = os.path.join(td, "h.cpp")
+        open(f, "w").write(src)
+        r = subprocess.run(["/opt/rocm/bin/hipcc", "--offload-arch=gfx950", "-std=c++17", f"-I{ROOT}/include", f,
+                            f"-L{os.path.dirname(SHIM_LIB)}", "-llaunchUpdateGrid", f"-L{P}", "-ltcsg",
+                            "-o", os.path.join(td, "h")], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr[:3000]
+
+
+def test_geometry_check_via_c_abi():
+    from threadcoarsenedstencilgenerator_amd import manifest
+    from threadcoarsenedstencilgenerator_amd import runtime as rt
+    L = rt.library()
+    d = manifest.cfg2_desc(2, 4, 128, shape="dppw")
+    g = rt.geometry(d, (520, 520, 520))
+    assert L.tcsg_geometry_check(C.byref(rt.to_cdesc(d)), C.byref(g)) == 0
+    g.right_offset += 1
+    assert L.tcsg_geometry_check(C.byref(rt.to_cdesc(d)), C.byref(g)) == 1
+    # a STREAM kernel's 32-bit buffer offsets cannot address a 4 GiB grid
+    big = rt.geometry(d, (1032, 1032, 520))
+    assert L.tcsg_geometry_check(C.byref(rt.to_cdesc(d)), C.byref(big)) == 5
+    # grids smaller than the halo are rejected
+    with pytest.raises(rt.TcsgError):
+        rt.geometry(d, (8, 520, 520))

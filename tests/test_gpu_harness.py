@@ -1,0 +1,54 @@
+"""The C++ `bench` harness (reference argv and TSV output) and a harness
+written against the reference's launchUpdateGrid.h, on the GPU."""
+import os
+import subprocess
+
+import pytest
+
+from threadcoarsenedstencilgenerator_amd import build, manifest
+
+pytestmark = pytest.mark.gpu
+
+
+def _bench(kernel_desc, *args):
+    env = dict(os.environ, TCSG_KERNEL=build.kernel_path(kernel_desc))
+    return subprocess.run([build.BENCH_BIN, *args], capture_output=True, text=True, env=env, timeout=120)
+
+
+def test_bench_sweep_output_format():
+    d = manifest.cfg1_desc("plain")  # 2D Box Jacobi r=1, 1x1, Row, None
+    r = _bench(d, "Box", "2D", "1", "Jacobi", "Row", "None", "0", "0.0", "1", "1", "130", "70")
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.strip().split("\n")
+    assert lines[0].startswith("#GPU: ")
+    assert "#grid size excluding border: x 128 y 68" in r.stdout
+    hdr = [l for l in lines if l.startswith("#xthreads")]
+    assert hdr and hdr[0].split("\t")[:3] == ["#xthreads", "ythreads", "zthreads"]
+    rows = [l.split("\t") for l in lines if not l.startswith("#")]
+    # every swept shape of 64..1024 threads (hip/benchmark.cpp:221-249), one step each
+    assert len(rows) == 230
+    assert all(row[2] == "0" and int(row[7]) == 1 and float(row[5]) > 0 for row in rows)
+
+
+def test_bench_single_block_and_mismatch():
+    d = manifest.cfg2_desc(2, 4, 128, shape="dppw", max_threads=256).replace(prefetch=0)
+    r = _bench(d, "Star", "3D", "4", "Variable", "Row", "Write", "16", "0.0", "2", "4", "128", "136", "72", "264",
+               "128", "2", "1")
+    assert r.returncode == 0, r.stderr
+    rows = [l for l in r.stdout.split("\n") if l and not l.startswith("#")]
+    assert len(rows) == 1 and rows[0].startswith("128\t2\t1\t")
+    # parameters that differ from the kernel's are refused (the reference's sanity check)
+    r = _bench(d, "Star", "3D", "3", "Variable", "Row", "Write", "16", "0.0", "2", "4", "128", "136", "72", "264")
+    assert r.returncode != 0 and "different stencil parameters" in r.stderr
+
+
+def test_launchupdategrid_harness():
+    d = manifest.cfg1_desc("plain")
+    env = dict(os.environ, TCSG_KERNEL=build.kernel_path(d))
+    r = subprocess.run([build.SHIM_DEMO, "Box", "Jacobi", "1", "2", "None", "258", "130"], capture_output=True,
+                       text=True, env=env, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "steps 1" in r.stdout
+    r = subprocess.run([build.SHIM_DEMO, "Star", "Jacobi", "1", "2", "None", "258", "130"], capture_output=True,
+                       text=True, env=env, timeout=60)
+    assert r.returncode == 3  # sanity check throws hipErrorInvalidValue

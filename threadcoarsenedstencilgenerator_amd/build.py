@@ -20,6 +20,7 @@ KDIR = os.path.join(PKG, "kernels")
 LIB = os.path.join(PKG, "libtcsg.so")
 BENCH_BIN = os.path.join(PKG, "harness", "bench")
 SHIM_LIB = os.path.join(PKG, "harness", "liblaunchUpdateGrid.so")
+SHIM_DEMO = os.path.join(PKG, "harness", "shim_demo")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -101,5 +102,8 @@ def build_harness() -> str:
           "-o", BENCH_BIN, f"-L{PKG}", "-ltcsg", f"-Wl,-rpath,{PKG}", "-Wl,-rpath,$ORIGIN/.."])
     _run([HIPCC, "-O2", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC", "-shared", f"-I{INCLUDE}",
           os.path.join(hd, "launchUpdateGrid.cpp"), "-o", SHIM_LIB, f"-L{PKG}", "-ltcsg",
+          "-Wl,-rpath,$ORIGIN/.."])
+    _run([HIPCC, "-O2", "-std=c++17", f"--offload-arch={ARCH}", f"-I{INCLUDE}", os.path.join(hd, "shim_demo.cpp"),
+          "-o", SHIM_DEMO, f"-L{hd}", "-llaunchUpdateGrid", f"-L{PKG}", "-ltcsg", "-Wl,-rpath,$ORIGIN",
           "-Wl,-rpath,$ORIGIN/.."])
     return BENCH_BIN

@@ -1,0 +1,159 @@
+"""Command lines of the generator, with the reference's positional argv.
+
+    generate2DStencil.py  Box|Star Row|Column Jacobi|Variable range xC yC [None|Read|Write CLsize]
+    generate3DStencil.py  Box|Star Row|Column Jacobi|Variable range xC yC zC [None|Read|Write CLsize]
+    generateLaunchUpdateGrid.py  Box|Star Jacobi|Variable range 2D|3D [None|Read|Write]
+
+(hip/generate2DStencil_hip.py:28-60, hip/generate3DStencil_hip.py:29-63,
+hip/generateLaunchUpdateGrid.py:24-45.)  The kernel generators print gfx950
+HIP source on stdout, as the reference prints kernels.cu; `-o FILE.co`
+compiles it to a code object instead.  Options that select the MI355X code
+shape (no reference counterpart) are given as --name=value:
+--variant=plain|stream, --xext=load|dpp|dppw, --dtype=f64|f32c,
+--max-threads=N, --prefetch=N, --waves-per-eu=N, --comp-group=N,
+--no-xcd-remap, --no-nt-store, --zfirst.
+Invalid arguments print the usage on stderr and exit with status 1, as the
+reference does.
+"""
+from __future__ import annotations
+
+import sys
+
+from .. import spec
+
+USAGE2 = ("generate2DStencil.py usage:\n\tstencil neighbourhood\t\t'Box' or 'Star'\n\taccess orientation\t'Row' or "
+          "'Column'\n\tcoefficients\t'Jacobi' or 'Variable'\n\trange\n\tcoarsening-factor in x\n\tcoarsening-factor in "
+          "y\nx-/y coarsening-factors[>0] are elements processed by a single thread\n\n\toptional:\n\talignment\t\t"
+          "'Write' or 'Read', default is 'None'\n\tsize of alignment (in units of grid-points)\n")
+USAGE3 = USAGE2.replace("generate2DStencil.py", "generate3DStencil.py").replace(
+    "\tcoarsening-factor in y\n", "\tcoarsening-factor in y\n\tcoarsening-factor in z\n")
+USAGE_L = ("generateLaunchUpdateGrid.py usage:\n\tstencil neighbourhood\t\teither 'Box' or 'Star'\n\tcoefficients\t\t"
+           "'Jacobi' or 'Variable'(coefficients have to be set by host)\n\trange\n\tdimension\n\n\toptional:\n"
+           "\talignment\t\t'Write' or 'Read'\n")
+
+
+def _err(msg, usage):
+    print(msg + "\n" + usage, file=sys.stderr)
+    sys.exit(1)
+
+
+def _split(argv):
+    """Positional arguments, --name=value options and the -o output path."""
+    opts = {}
+    out = None
+    i = 0
+    pos = []
+    while i < len(argv):
+        a = argv[i]
+        if a == "-o":
+            out = argv[i + 1]
+            i += 2
+            continue
+        if a.startswith("--"):
+            k, _, v = a[2:].partition("=")
+            opts[k] = v
+        else:
+            pos.append(a)
+        i += 1
+    return pos, opts, out
+
+
+def _int(s, what, usage):
+    try:
+        return int(s)
+    except ValueError:
+        _err(f"{what} must be an integer", usage)
+
+
+def desc_from_argv(dim: int, argv):
+    usage = USAGE2 if dim == 2 else USAGE3
+    pos, opts, out = _split(argv)
+    nfix = 6 if dim == 2 else 7
+    if len(pos) not in (nfix, nfix + 2):
+        _err("Unknown arguments!", usage)
+    nb, orient, coeff = pos[0], pos[1], pos[2]
+    r = _int(pos[3], "range", usage)
+    c = [_int(v, "coarsening factor", usage) for v in pos[4:4 + dim - 1 + 1]]
+    align, cl = "None", 0
+    if len(pos) == nfix + 2:
+        align, cl = pos[nfix], _int(pos[nfix + 1], "size of alignment", usage)
+        if cl <= 0 and align != "None":
+            _err("size of alignment has to be > 0", usage)
+    if nb not in spec.NB_NAMES:
+        _err("Unknown stencil neighbourhood!", usage)
+    if orient not in spec.ORIENT_NAMES:
+        _err("Unknown access orientation!", usage)
+    if coeff not in spec.COEFF_NAMES:
+        _err("Unknown coefficient type!", usage)
+    if r < 1:
+        _err("Range has to be at least 1!", usage)
+    if min(c) < 1:
+        _err("x- or y-coarsening factors have to be at least 1!", usage)
+    if align not in spec.ALIGN_NAMES:
+        _err("Unkown alignment! Default is 'None'.", usage)
+    kw = dict(dimension=dim, neighbourhood=spec.NB_NAMES[nb], orientation=spec.ORIENT_NAMES[orient],
+              coeff=spec.COEFF_NAMES[coeff], range=r, xc=c[0], yc=c[1], zc=c[2] if dim == 3 else 1,
+              alignment=spec.ALIGN_NAMES[align], cl_size=cl if align != "None" else 0,
+              variant=spec.VARIANT_NAMES[opts.get("variant", "stream")],
+              dtype=spec.DTYPE_NAMES[opts.get("dtype", "f64")],
+              xext=opts.get("xext", "dpp"), xcd_remap="no-xcd-remap" not in opts, nt_store="no-nt-store" not in opts,
+              zfirst="zfirst" in opts)
+    for k, f in (("max-threads", "max_threads"), ("prefetch", "prefetch"), ("waves-per-eu", "waves_per_eu"),
+                 ("comp-group", "comp_group")):
+        if k in opts:
+            kw[f] = _int(opts[k], k, usage)
+    if kw["variant"] == spec.PLAIN:
+        kw.update(xext="load", xcd_remap=False, nt_store=False, zfirst=False)
+    try:
+        return spec.StencilDesc(**kw), out
+    except ValueError as e:
+        _err(str(e), usage)
+
+
+def generate(dim: int, argv=None) -> int:
+    from .emit import emit_source
+    d, out = desc_from_argv(dim, sys.argv[1:] if argv is None else argv)
+    if out:
+        from ..build import HIP_FLAGS, HIPCC, _run
+        import os
+        import tempfile
+        with tempfile.TemporaryDirectory() as td:
+            src = os.path.join(td, "kernels.hip")
+            open(src, "w").write(emit_source(d))
+            _run([HIPCC, *HIP_FLAGS, "--genco", "--no-gpu-bundle-output", src, "-o", out])
+    else:
+        sys.stdout.write(emit_source(d))
+    return 0
+
+
+def generate_launch(argv=None) -> int:
+    """The reference emits launchUpdateGrid.cu, which compiles the kernels
+    into the harness and checks the parameters baked into them.  Here the
+    kernels are code objects loaded at run time, so this prints a C++ unit
+    that builds the launchUpdateGrid.h entry points over libtcsg with the
+    expected parameters fixed, for harnesses written against that header."""
+    argv = sys.argv[1:] if argv is None else argv
+    if len(argv) not in (4, 5):
+        _err("Unknown arguments!", USAGE_L)
+    nb, coeff, r, dim = argv[0], argv[1], _int(argv[2], "range", USAGE_L), argv[3]
+    align = argv[4] if len(argv) == 5 else "None"
+    if nb not in spec.NB_NAMES:
+        _err("Unknown stencil neighbourhood!", USAGE_L)
+    if coeff not in spec.COEFF_NAMES:
+        _err("Unknown coefficient-type!", USAGE_L)
+    if r < 1:
+        _err("Range has to be at least 1!", USAGE_L)
+    if dim not in ("2D", "3D"):
+        _err("Unknown dimension!", USAGE_L)
+    if align not in spec.ALIGN_NAMES:
+        _err("Unknown alignment!", USAGE_L)
+    import os
+    shim = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "harness", "launchUpdateGrid.cpp")
+    print("// Generated by threadcoarsenedstencilgenerator_amd.generator (launch interface).")
+    print(f'#define TCSG_EXPECT_NEIGHBOURHOOD "{nb}"')
+    print(f'#define TCSG_EXPECT_COEFF "{coeff}"')
+    print(f"#define TCSG_EXPECT_RANGE {r}")
+    print(f"#define TCSG_EXPECT_DIMENSION {dim[0]}")
+    print(f'#define TCSG_EXPECT_ALIGNMENT "{align}"')
+    print(f'#include "{shim}"')
+    return 0

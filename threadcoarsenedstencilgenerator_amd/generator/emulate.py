@@ -17,7 +17,8 @@ from ..spec import F32C
 from .emit import emit_source
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-EMU_DIR = os.path.join(os.path.dirname(HERE), "kernels", "emu")
+# host-only build cache (the emulation libraries never run on the GPU)
+EMU_DIR = os.path.join(tempfile.gettempdir(), f"tcsg_emu_{os.getuid()}")
 
 
 class _KArgs(C.Structure):

@@ -54,6 +54,11 @@ std::vector<double> benchmarkUpdateGrid(std::string coeffType, std::string stenc
     bool ok = coeffType == (d.coeff == TCSG_JACOBI ? "Jacobi" : "Variable") &&
               stencilType == (d.neighbourhood == TCSG_BOX ? "Box" : "Star") && stencilRange == d.range &&
               dimension == d.dimension && alignment == al[d.alignment] && d.dtype == TCSG_F64;
+#ifdef TCSG_EXPECT_RANGE
+    // parameters fixed by generateLaunchUpdateGrid.py (hip/generateLaunchUpdateGrid.py:106-119)
+    ok = ok && coeffType == TCSG_EXPECT_COEFF && stencilType == TCSG_EXPECT_NEIGHBOURHOOD &&
+         stencilRange == TCSG_EXPECT_RANGE && dimension == TCSG_EXPECT_DIMENSION && alignment == TCSG_EXPECT_ALIGNMENT;
+#endif
     tcsg_geometry g;
     if (!ok || !geometry_of(d, gridSize, rightOffset, g)) throw(hipErrorInvalidValue);
     std::vector<double> dur(1 << 16);

@@ -54,7 +54,7 @@ def test_f32c_error_vs_f64_after_20_steps():
     """FP32 storage, compensated accumulation: error of the 20-step state
     against the FP64 result of the same workload (which the test above ties
     bit-for-bit to the oracle), computed in FP64 on the GPU.  Tolerance
-    normwise max |u32 - u64| / max |u64| <= 1e-6."""
+    (the accuracy bound for this variant): normwise max |u32 - u64| / max |u64| <= 1e-6."""
     import torch
 
     from threadcoarsenedstencilgenerator_amd import runtime as rt

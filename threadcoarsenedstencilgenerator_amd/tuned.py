@@ -14,7 +14,8 @@ CFG2_F64 = dict(desc=manifest.cfg2_desc(2, 4, 128, shape="dppw", max_threads=256
                 block=(128, 2, 1))
 # the same workload in FP32 storage with compensated accumulation
 # (x coarsening 2: FP32 kernels with 4 points per lane in x returned wrong values
-# in lanes 12-15 of each 16-lane group on the test machines)
+# in lanes 12-15 of each 16-lane group on the test machines; the xc=2 kernels and all
+# FP64 kernels are bit-exact over the full grid)
 CFG2_F32C = dict(desc=manifest.cfg2_desc(2, 8, 64, shape="dppw", max_threads=256, dtype=F32C).replace(
     prefetch=0, comp_group=13), block=(128, 1, 1))
 

@@ -155,6 +155,29 @@ int tcsg_benchmark_update_grid(tcsg_kernel *k, const tcsg_geometry *g, tcsg_dim3
 int tcsg_time_steps(tcsg_kernel *k, const tcsg_geometry *g, tcsg_dim3 block, tcsg_dim3 blocks,
                     void *d0, void *d1, int warmup, int steps, void *stream, float *ms);
 
+/* ---- batches of independent grids (kernels generated with the batched
+ * option; no reference counterpart) ----
+ * One persistent kernel per time step advances every grid of the batch: its
+ * workgroups stride over a tile table whose entries are (grid, block) pairs,
+ * each the work of one block of tcsg_update.  geoms[i] is grid i's
+ * geometry, bufs0[i]/bufs1[i] its ping-pong buffers; every tile uses `block`;
+ * workgroups = 0 picks 8 per compute unit (at most one per tile). */
+typedef struct tcsg_batch tcsg_batch;
+int tcsg_batch_create(tcsg_batch **out, tcsg_kernel *k, uint32_t n, const tcsg_geometry *geoms,
+                      void *const *bufs0, void *const *bufs1, tcsg_dim3 block, uint32_t workgroups);
+int tcsg_batch_free(tcsg_batch *b);
+/* `steps` steps of every grid starting from bufs0 (step t reads buffer t%2),
+ * one persistent launch per step.  use_graph != 0: the launches are captured
+ * once into a hipGraph (kept per step count) and the graph is replayed.
+ * Asynchronous on `stream` (NULL: the library's own stream, synchronised
+ * before returning).  *final_index = steps % 2. */
+int tcsg_batch_run(tcsg_batch *b, int steps, int use_graph, void *stream, int *final_index);
+/* The same steps with every grid launched separately (tcsg_update_grid per
+ * grid per step) -- the baseline the persistent kernel replaces. */
+int tcsg_batch_run_separate(tcsg_batch *b, int steps, void *stream, int *final_index);
+/* Total interior points of the batch and its tile count. */
+int tcsg_batch_info(const tcsg_batch *b, uint64_t *interior_points, uint32_t *tiles, uint32_t *workgroups);
+
 /* Error of an F32C state against an FP64 state over the interior, computed
  * in FP64 on the GPU: out[0] = max |a-b|, out[1] = max |b|,
  * out[2] = sum (a-b)^2, out[3] = sum b^2, out[4] = max |a-b|/max(|b|, floor).

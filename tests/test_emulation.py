@@ -58,3 +58,30 @@ def test_emulated_stream_kernel_matches_oracle(case):
         ui = np.uint32 if dt == np.float32 else np.uint64
         diff = np.flatnonzero(got.view(ui) != ref.view(ui))
         assert diff.size == 0, f"block {block}: {diff.size} points differ, first {diff[:5]}"
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32c"])
+def test_emulated_batched_kernel_matches_oracle(dtype):
+    """The persistent kernel over a tile table of uneven grids (fewer
+    workgroups than tiles, so each workgroup loops) equals the oracle per grid."""
+    from threadcoarsenedstencilgenerator_amd import runtime as rt
+    d = manifest.cfg4_desc(shape="dppw", batched=True, dtype=F32C if dtype == "f32c" else spec.F64)
+    sizes = rt.batch_sizes(4, 7, 8, 24, d.coarsening, d.range)
+    dt = np.float32 if d.dtype == F32C else np.float64
+    geoms, a0, a1, refs = [], [], [], []
+    run = oracle.run_f32c if d.dtype == F32C else oracle.run_f64
+    for i, size in enumerate(sizes):
+        geo = spec.geometry(d, size)
+        og = oracle.Geometry(3, d.range, d.alignment, d.cl_size, size, d.coarsening)
+        a = np.zeros(og.n_points)
+        oracle.init_seeded(og, 0, 10 + i, a)
+        a = a.astype(dt)
+        geoms.append(geo)
+        a0.append(a.copy())
+        a1.append(a.copy())
+        refs.append(run(d, og, None, a.copy(), a.copy(), 3))
+    got, bad = emulate.run_batched(d, geoms, a0, a1, None, 3, (64, 2, 1), 5)
+    assert bad == 0
+    for g, r in zip(got, refs):
+        ui = np.uint32 if dt == np.float32 else np.uint64
+        assert np.array_equal(g.view(ui), r.view(ui))

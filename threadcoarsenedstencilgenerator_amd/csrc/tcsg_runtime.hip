@@ -522,6 +522,188 @@ int tcsg_time_steps(tcsg_kernel *k, const tcsg_geometry *g, tcsg_dim3 block, tcs
     return TCSG_OK;
 }
 
+} // extern "C"
+
+namespace {
+struct Tile {
+    uint32_t grid, bx, by, bz;
+};
+struct BatchArgs {
+    const KArgs *grids;
+    const Tile *tiles;
+    uint32_t ntiles, pad;
+};
+static_assert(sizeof(BatchArgs) == 24, "batch argument layout");
+} // namespace
+
+struct tcsg_batch {
+    tcsg_kernel *k = nullptr;
+    hipFunction_t fn = nullptr;
+    std::vector<tcsg_geometry> geoms;
+    std::vector<tcsg_dim3> blocks;
+    std::vector<void *> b0, b1;
+    tcsg_dim3 block{};
+    uint32_t ntiles = 0, nwg = 0;
+    uint64_t points = 0;
+    KArgs *d_grids[2] = {nullptr, nullptr};
+    Tile *d_tiles = nullptr;
+    hipStream_t own = nullptr;
+    std::vector<std::pair<int, hipGraphExec_t>> graphs;
+};
+
+extern "C" {
+
+int tcsg_batch_free(tcsg_batch *b)
+{
+    if (!b) return TCSG_OK;
+    for (auto &g : b->graphs) (void)hipGraphExecDestroy(g.second);
+    for (auto *p : b->d_grids)
+        if (p) (void)hipFree(p);
+    if (b->d_tiles) (void)hipFree(b->d_tiles);
+    if (b->own) (void)hipStreamDestroy(b->own);
+    delete b;
+    return TCSG_OK;
+}
+
+int tcsg_batch_create(tcsg_batch **out, tcsg_kernel *k, uint32_t n, const tcsg_geometry *geoms, void *const *bufs0,
+                      void *const *bufs1, tcsg_dim3 block, uint32_t workgroups)
+{
+    if (!out || !k || !n || !geoms || !bufs0 || !bufs1) return TCSG_ERR_ARG;
+    *out = nullptr;
+    if (k->desc.variant != TCSG_STREAM) return TCSG_ERR_MISMATCH;
+    int rc = tcsg_check_block(k, block);
+    if (rc) return rc;
+    if (k->desc.coeff == TCSG_VARIABLE && !k->mask_set) return TCSG_ERR_NO_MASK;
+    auto *b = new tcsg_batch();
+    b->k = k;
+    b->block = block;
+    if (hipModuleGetFunction(&b->fn, k->module, "tcsg_update_batched") != hipSuccess) {
+        (void)hipGetLastError();
+        delete b;
+        return TCSG_ERR_MISMATCH;  // code object generated without the batched kernel
+    }
+    std::vector<KArgs> args[2];
+    std::vector<Tile> tiles;
+    const uint64_t es = elem_size(k->desc);
+    for (uint32_t i = 0; i < n; ++i) {
+        rc = check_geom(k->desc, &geoms[i]);
+        if (!rc && (!bufs0[i] || !bufs1[i] || bufs0[i] == bufs1[i])) rc = TCSG_ERR_ARG;
+        if (rc) { delete b; return rc; }
+        const tcsg_dim3 nb = tcsg_grid_blocks(&geoms[i], block);
+        b->geoms.push_back(geoms[i]);
+        b->blocks.push_back(nb);
+        b->b0.push_back(bufs0[i]);
+        b->b1.push_back(bufs1[i]);
+        b->points += (uint64_t)geoms[i].elements.x * geoms[i].elements.y * geoms[i].elements.z;
+        for (int p = 0; p < 2; ++p) {
+            KArgs a{};
+            a.in = p ? bufs1[i] : bufs0[i];
+            a.out = p ? bufs0[i] : bufs1[i];
+            a.mask = k->mask;
+            a.xdim = geoms[i].size.x;
+            a.ydim = geoms[i].size.y;
+            a.zdim = geoms[i].size.z;
+            a.right_offset = geoms[i].right_offset;
+            a.in_bytes = a.out_bytes = (uint32_t)(geoms[i].n_points * es);
+            args[p].push_back(a);
+        }
+        for (uint32_t z = 0; z < nb.z; ++z)
+            for (uint32_t y = 0; y < nb.y; ++y)
+                for (uint32_t x = 0; x < nb.x; ++x) tiles.push_back({i, x, y, z});
+    }
+    b->ntiles = (uint32_t)tiles.size();
+    uint32_t nwg = workgroups;
+    if (!nwg) {
+        int dev = 0, cus = 256;
+        if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        nwg = 8u * (uint32_t)cus;
+    }
+    b->nwg = std::min(nwg, b->ntiles);
+    hipError_t e = hipSuccess;
+    for (int p = 0; p < 2 && e == hipSuccess; ++p) {
+        e = hipMalloc(&b->d_grids[p], n * sizeof(KArgs));
+        if (e == hipSuccess) e = hipMemcpy(b->d_grids[p], args[p].data(), n * sizeof(KArgs), hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess) e = hipMalloc(&b->d_tiles, tiles.size() * sizeof(Tile));
+    if (e == hipSuccess) e = hipMemcpy(b->d_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->own, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        g_last_hip = (int)e;
+        tcsg_batch_free(b);
+        return TCSG_ERR_HIP;
+    }
+    *out = b;
+    return TCSG_OK;
+}
+
+static int batch_launch(tcsg_batch *b, int parity, hipStream_t s)
+{
+    BatchArgs a{b->d_grids[parity], b->d_tiles, b->ntiles, 0};
+    size_t sz = sizeof(a);
+    void *extra[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, &a, HIP_LAUNCH_PARAM_BUFFER_SIZE, &sz, HIP_LAUNCH_PARAM_END};
+    TCSG_HIP(hipModuleLaunchKernel(b->fn, b->nwg, 1, 1, b->block.x, b->block.y, b->block.z, 0, s, nullptr, extra));
+    return TCSG_OK;
+}
+
+int tcsg_batch_run(tcsg_batch *b, int steps, int use_graph, void *stream, int *final_index)
+{
+    if (!b || steps < 0) return TCSG_ERR_ARG;
+    const hipStream_t s = stream ? (hipStream_t)stream : b->own;
+    if (!use_graph) {
+        for (int t = 0; t < steps; ++t) {
+            const int rc = batch_launch(b, t & 1, s);
+            if (rc) return rc;
+        }
+    } else if (steps > 0) {
+        hipGraphExec_t exec = nullptr;
+        for (auto &g : b->graphs)
+            if (g.first == steps) exec = g.second;
+        if (!exec) {
+            // capture on the library's stream, then replay on the caller's
+            TCSG_HIP(hipStreamBeginCapture(b->own, hipStreamCaptureModeThreadLocal));
+            int rc = TCSG_OK;
+            for (int t = 0; t < steps && !rc; ++t) rc = batch_launch(b, t & 1, b->own);
+            hipGraph_t graph = nullptr;
+            const hipError_t e = hipStreamEndCapture(b->own, &graph);
+            if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
+            TCSG_HIP(e);
+            const hipError_t e2 = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            TCSG_HIP(e2);
+            b->graphs.emplace_back(steps, exec);
+        }
+        TCSG_HIP(hipGraphLaunch(exec, s));
+    }
+    if (!stream) TCSG_HIP(hipStreamSynchronize(b->own));
+    if (final_index) *final_index = steps & 1;
+    return TCSG_OK;
+}
+
+int tcsg_batch_run_separate(tcsg_batch *b, int steps, void *stream, int *final_index)
+{
+    if (!b || steps < 0) return TCSG_ERR_ARG;
+    const hipStream_t s = stream ? (hipStream_t)stream : b->own;
+    for (int t = 0; t < steps; ++t)
+        for (size_t i = 0; i < b->geoms.size(); ++i) {
+            void *in = (t & 1) ? b->b1[i] : b->b0[i];
+            void *out = (t & 1) ? b->b0[i] : b->b1[i];
+            const int rc = launch_one(b->k, &b->geoms[i], b->block, b->blocks[i], in, out, s);
+            if (rc) return rc;
+        }
+    if (!stream) TCSG_HIP(hipStreamSynchronize(b->own));
+    if (final_index) *final_index = steps & 1;
+    return TCSG_OK;
+}
+
+int tcsg_batch_info(const tcsg_batch *b, uint64_t *points, uint32_t *tiles, uint32_t *workgroups)
+{
+    if (!b) return TCSG_ERR_ARG;
+    if (points) *points = b->points;
+    if (tiles) *tiles = b->ntiles;
+    if (workgroups) *workgroups = b->nwg;
+    return TCSG_OK;
+}
+
 int tcsg_error_f32_vs_f64(const tcsg_geometry *g, const float *a, const double *b, double floor_v, double *out)
 {
     if (!g || !a || !b || !out) return TCSG_ERR_ARG;

@@ -75,6 +75,7 @@ class _Emitter:
     def __init__(self, d: StencilDesc, target: str = "gfx950"):
         self.d = d
         self.target = target
+        self.batched_body = False
         self.f32 = d.dtype == F32C
         self.T = "float" if self.f32 else "double"
         self.R = d.range
@@ -267,11 +268,33 @@ class _Emitter:
         w("  }")
         w("  return 0;")
         w("}")
+        if self.d.batched:
+            w('extern "C" int tcsg_emulate_batched(const void* grids, const void* tiles, uint32_t ntiles,')
+            w("                                    uint32_t nwg, uint32_t bx, uint32_t by, uint32_t bz) {")
+            w("  const uint32_t nt = bx * by * bz;")
+            w("  if (nt == 0 || nt % 64) return -1;")
+            w("  gridDim = {nwg, 1, 1};")
+            w("  blockDim = {bx, by, bz};")
+            w("  tcsg_batch_args ba{(const tcsg_kargs*)grids, (const tcsg_tile*)tiles, ntiles, 0};")
+            w("  for (uint32_t i = 0; i < nwg; ++i) {")
+            w("    blockIdx = {i, 0, 0};")
+            w("    for (uint32_t w0 = 0; w0 < nt; w0 += 64) {")
+            w("      for (int l = 0; l < 64; ++l) {")
+            w("        const uint32_t lid = w0 + l;")
+            w("        threadIdx.x.v[l] = lid % bx; threadIdx.y.v[l] = (lid / bx) % by; threadIdx.z.v[l] = lid / (bx * by);")
+            w("      }")
+            w("      tcsg_update_batched(ba);")
+            w("    }")
+            w("  }")
+            w("  return 0;")
+            w("}")
 
     # ------------------------------------------------------------ body parts
     def block_coords(self):
         w = self.w
-        if self.d.xcd_remap:
+        if self.batched_body:
+            w("  const u32 bxi = tl.bx, byi = tl.by, bzi = tl.bz;")
+        elif self.d.xcd_remap:
             w("  // XCD-aware remap: hardware hands consecutive workgroups to the 8 XCDs")
             w("  // round-robin; give each XCD a contiguous range of tiles instead.")
             w("  const u32 nbx = gridDim.x, nby = gridDim.y, nbz = gridDim.z;")
@@ -556,9 +579,32 @@ class _Emitter:
         else:
             self.body_stream()
         self.w("}")
+        if d.batched:
+            self.batched_kernel(attr)
         if self.target == "cpu":
             self.emulation_runner()
         return "\n".join(self.L) + "\n"
+
+    def batched_kernel(self, attr):
+        """Persistent kernel: the workgroups stride over a tile table whose
+        entries name a grid (its kernel arguments) and a block of that grid;
+        each tile is the work of one block of tcsg_update."""
+        w = self.w
+        d = self.d
+        w("")
+        w("struct tcsg_tile { uint32_t grid, bx, by, bz; };")
+        w("struct tcsg_batch_args { const tcsg_kargs* grids; const tcsg_tile* tiles; uint32_t ntiles, pad; };")
+        w(f'extern "C" __global__ void __launch_bounds__({d.max_threads}){attr} '
+          f"tcsg_update_batched(const tcsg_batch_args ba)")
+        w("{")
+        w("  for (uint32_t t = blockIdx.x; t < ba.ntiles; t += gridDim.x) {")
+        w("  const tcsg_tile tl = ba.tiles[t];")
+        w("  const tcsg_kargs a = ba.grids[tl.grid];")
+        self.batched_body = True
+        self.body_stream()
+        self.batched_body = False
+        w("  }")
+        w("}")
 
 
 # Straight-line kernels grow with outputs-per-thread x stencil size; past this

@@ -64,3 +64,32 @@ def run(desc: spec.StencilDesc, geom, mask, a: np.ndarray, b: np.ndarray, steps:
             raise ValueError("block shape must be whole waves")
     bad = C.c_ulong.in_dll(L, "tcsg_emu_bad_access").value
     return bufs[steps & 1], bad
+
+
+def run_batched(desc: spec.StencilDesc, geoms, arrays0, arrays1, mask, steps: int, block, nwg: int):
+    """Emulate tcsg_update_batched over several grids (tile table built as the
+    runtime builds it: grid-major, x fastest).  Returns (final arrays, bad)."""
+    L = C.CDLL(build(desc))
+    L.tcsg_emulate_batched.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                       C.c_uint32]
+    dt = np.float32 if desc.dtype == F32C else np.float64
+    m = np.ascontiguousarray(mask if mask is not None else np.zeros(1), dtype=dt)
+    tiles = []
+    for gi, g in enumerate(geoms):
+        nb = [(g.threads[a] + block[a] - 1) // block[a] for a in range(3)]
+        for z in range(nb[2]):
+            for y in range(nb[1]):
+                for x in range(nb[0]):
+                    tiles.append((gi, x, y, z))
+    tiles = np.array(tiles, dtype=np.uint32)
+    bufs = [list(arrays0), list(arrays1)]
+    for s in range(steps):
+        ka = (_KArgs * len(geoms))()
+        for i, g in enumerate(geoms):
+            src, dst = bufs[s & 1][i], bufs[(s + 1) & 1][i]
+            ka[i] = _KArgs(src.ctypes.data, dst.ctypes.data, m.ctypes.data, g.xdim, g.ydim, g.zdim, g.ro,
+                           src.nbytes, dst.nbytes, 0, 0)
+        if L.tcsg_emulate_batched(C.cast(ka, C.c_void_p), tiles.ctypes.data, len(tiles), nwg, *block) != 0:
+            raise ValueError("block shape must be whole waves")
+    bad = C.c_ulong.in_dll(L, "tcsg_emu_bad_access").value
+    return bufs[steps & 1], bad

@@ -118,10 +118,12 @@ def cfg3_desc(xc=1, yc=1, orientation=ROW, dtype=F64, shape="plain", max_threads
                        **SHAPES[shape])
 
 
-def cfg4_desc(dtype=F64, shape="dpp"):
-    """3D Box Jacobi r=2, coarsening 2x2x4."""
-    return StencilDesc(dimension=3, neighbourhood=BOX, orientation=ROW, coeff=JACOBI, range=2, xc=2, yc=2,
-                       zc=4, alignment=ALIGN_NONE, cl_size=0, dtype=dtype, **SHAPES[shape])
+def cfg4_desc(dtype=F64, shape="dpp", batched=False, max_threads=256):
+    """3D Box Jacobi r=2, coarsening 2x2x4 (the batched uneven-grids workload)."""
+    d = StencilDesc(dimension=3, neighbourhood=BOX, orientation=ROW, coeff=JACOBI, range=2, xc=2, yc=2,
+                    zc=4, alignment=ALIGN_NONE, cl_size=0, dtype=dtype, max_threads=max_threads,
+                    **SHAPES[shape])
+    return d.replace(batched=batched) if batched else d
 
 
 def default_build_set():
@@ -132,6 +134,8 @@ def default_build_set():
     descs += [cfg1_desc("plain"), cfg1_desc("stream"), cfg1_desc("dppw")]
     descs += [cfg2_desc(1, 1, 1), cfg2_desc(2, 4, 8)]            # reference-shaped kernels
     descs += [cfg4_desc(F64, "plain"), cfg4_desc(F64, "dppw")]
+    descs += [cfg4_desc(F64, "dppw", batched=True), cfg4_desc(F32C, "dppw", batched=True),
+              cfg4_desc(F32C, "plain")]
     from . import tuned
     descs += tuned.build_set()
     return list(dict.fromkeys(descs))

@@ -84,6 +84,12 @@ def _lib():
                                                  C.POINTER(C.c_double), C.c_size_t, C.POINTER(C.c_size_t)]
         L.tcsg_time_steps.argtypes = [p, C.POINTER(CGeometry), Dim3, Dim3, p, p, i, i, p,
                                       C.POINTER(C.c_float)]
+        L.tcsg_batch_create.argtypes = [C.POINTER(p), p, u32, C.POINTER(CGeometry), C.POINTER(p), C.POINTER(p),
+                                        Dim3, u32]
+        L.tcsg_batch_free.argtypes = [p]
+        L.tcsg_batch_run.argtypes = [p, i, i, p, C.POINTER(i)]
+        L.tcsg_batch_run_separate.argtypes = [p, i, p, C.POINTER(i)]
+        L.tcsg_batch_info.argtypes = [p, C.POINTER(u64), C.POINTER(u32), C.POINTER(u32)]
         L.tcsg_error_f32_vs_f64.argtypes = [C.POINTER(CGeometry), p, p, C.c_double, C.POINTER(C.c_double)]
         L.tcsg_convert_f64_to_f32.argtypes = [p, p, u64, p]
         L.tcsg_status_string.argtypes = [i]
@@ -250,3 +256,69 @@ class Grids:
     def upload(self, i, arr: np.ndarray):
         import torch
         self.bufs[i].copy_(torch.from_numpy(np.ascontiguousarray(arr)))
+
+
+class Batch:
+    """Many independent grids advanced by one persistent kernel per step
+    (the kernel must be generated with ``batched=True``)."""
+
+    def __init__(self, kernel: Kernel, geoms, bufs0, bufs1, block, workgroups: int = 0):
+        n = len(geoms)
+        G = (CGeometry * n)(*geoms)
+        P = C.c_void_p * n
+        h = C.c_void_p()
+        _check(_lib().tcsg_batch_create(C.byref(h), kernel._h, n, G, P(*bufs0), P(*bufs1), dim3(block),
+                                        workgroups), "batch_create")
+        self._h = h
+        self.kernel = kernel  # keeps the module alive
+
+    def run(self, steps: int, graph: bool = True, stream=None) -> int:
+        fin = C.c_int()
+        _check(_lib().tcsg_batch_run(self._h, steps, int(graph), _stream(stream), C.byref(fin)), "batch_run")
+        return fin.value
+
+    def run_separate(self, steps: int, stream=None) -> int:
+        fin = C.c_int()
+        _check(_lib().tcsg_batch_run_separate(self._h, steps, _stream(stream), C.byref(fin)), "batch_run_separate")
+        return fin.value
+
+    def info(self):
+        pts, tiles, wg = C.c_uint64(), C.c_uint32(), C.c_uint32()
+        _check(_lib().tcsg_batch_info(self._h, C.byref(pts), C.byref(tiles), C.byref(wg)), "batch_info")
+        return pts.value, tiles.value, wg.value
+
+    def close(self):
+        if self._h:
+            _lib().tcsg_batch_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def batch_sizes(n: int, seed: int, lo: int, hi: int, coarsening, r: int):
+    """Grid sizes (incl. halo) of the batched workload: each interior axis
+    length log-uniform in [lo, hi] from SplitMix64(seed), rounded to a
+    multiple of the coarsening factor."""
+    import math
+    M = (1 << 64) - 1
+    out = []
+    k = 0
+    for _ in range(n):
+        size = []
+        for a in range(3):
+            z = (seed + (k + 1) * 0x9E3779B97F4A7C15) & M
+            z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+            z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+            z ^= z >> 31
+            k += 1
+            u = (z >> 11) * 2.0 ** -53
+            length = math.exp(math.log(lo) + u * (math.log(hi) - math.log(lo)))
+            c = coarsening[a]
+            length = max(c, int(round(length / c)) * c)
+            size.append(length + 2 * r)
+        out.append(tuple(size))
+    return out

@@ -36,7 +36,7 @@ class StencilDesc:
     dimension/neighbourhood/orientation/coeff/range/coarsening/alignment/
     cl_size mirror the reference generator argv.  dtype, variant and the
     tuning knobs (max_threads, xext, xcd_remap, nt_store, comp_group,
-    prefetch, waves_per_eu, zfirst) have no reference counterpart: they select the
+    prefetch, waves_per_eu, zfirst, batched) have no reference counterpart: they select the
     MI355X code shape.
     """
 
@@ -63,6 +63,8 @@ class StencilDesc:
     waves_per_eu: int = 0    # STREAM: occupancy the compiler must allow (0: its own choice)
     zfirst: bool = False     # with xcd_remap: order tiles z-fastest, so z-neighbour tiles (which
     #                          share halo planes) run together on one XCD
+    batched: bool = False    # STREAM: also emit tcsg_update_batched, a persistent kernel over a
+    #                          tile table spanning many grids
 
     def __post_init__(self):
         if self.dimension not in (2, 3):
@@ -85,6 +87,8 @@ class StencilDesc:
             raise ValueError("size of alignment has to be > 0")
         if self.xext not in ("load", "dpp", "dppw"):
             raise ValueError("xext must be 'load', 'dpp' or 'dppw'")
+        if self.batched and self.variant != STREAM:
+            raise ValueError("the batched kernel is a STREAM variant")
         if self.prefetch < 0 or self.waves_per_eu < 0 or self.waves_per_eu > 8:
             raise ValueError("prefetch must be >= 0 and waves_per_eu in [0, 8]")
         if self.max_threads % 64 or not 64 <= self.max_threads <= 1024:
@@ -137,7 +141,7 @@ class StencilDesc:
                 self.coeff, self.range, self.xc, self.yc, self.zc, self.alignment,
                 self.cl_size, self.dtype, self.variant, self.n, self.max_threads,
                 {"load": 0, "dpp": 1, "dppw": 2}[self.xext], int(self.xcd_remap), int(self.nt_store),
-                self.comp_group, self.prefetch, self.waves_per_eu, int(self.zfirst), 0]
+                self.comp_group, self.prefetch, self.waves_per_eu, int(self.zfirst), int(self.batched)]
 
     def replace(self, **kw) -> "StencilDesc":
         return dataclasses.replace(self, **kw)

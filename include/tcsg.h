@@ -188,6 +188,35 @@ int tcsg_error_f32_vs_f64(const tcsg_geometry *g, const float *a, const double *
 /* FP64 -> FP32 copy of a whole grid (round to nearest).  Asynchronous. */
 int tcsg_convert_f64_to_f32(const double *src, float *dst, uint64_t n, void *stream);
 
+/* ---- cost-model network of the learned block-shape tuner (no reference
+ * counterpart: the reference measures every block shape, hip/benchmark.cpp:
+ * 221-263; the tuner ranks them with this network and measures the top few).
+ * A 3-layer MLP  y = W3 relu(W2 relu(W1 x + b1) + b2) + b3  in FP32 with
+ * `hidden` units per hidden layer, trained on a mean-squared error with Adam
+ * (beta1 0.9, beta2 0.999, eps 1e-8, bias-corrected).  Parameters are one
+ * array in the order W1[hidden][in], b1, W2[hidden][hidden], b2, W3[hidden],
+ * b3.  Device pointers; the network owns its parameters and activations. */
+typedef struct tcsg_mlp tcsg_mlp;
+size_t tcsg_mlp_num_params(uint32_t in_dim, uint32_t hidden);
+/* He-uniform weights drawn from `seed` (SplitMix64), zero biases. */
+int tcsg_mlp_create(tcsg_mlp **out, uint32_t in_dim, uint32_t hidden, uint32_t max_batch, uint64_t seed);
+int tcsg_mlp_free(tcsg_mlp *n);
+/* Host copies of the parameters; set also resets the Adam state. */
+int tcsg_mlp_get_params(const tcsg_mlp *n, float *host, size_t count);
+int tcsg_mlp_set_params(tcsg_mlp *n, const float *host, size_t count);
+/* d_out[b] = network(d_x[b][0..in)) for b < batch.  Asynchronous on stream. */
+int tcsg_mlp_forward(tcsg_mlp *n, const float *d_x, uint32_t batch, float *d_out, void *stream);
+/* Gradient of the mean-squared error over the batch with respect to every
+ * parameter (host copy, same layout) and the loss.  Synchronous; the
+ * parameters are unchanged. */
+int tcsg_mlp_gradients(tcsg_mlp *n, const float *d_x, const float *d_y, uint32_t batch, float *host_grads,
+                       float *host_loss);
+/* One Adam step on the batch's mean-squared error with learning rate lr;
+ * *d_loss (device, optional) receives the loss before the step.
+ * Asynchronous on stream. */
+int tcsg_mlp_train_step(tcsg_mlp *n, const float *d_x, const float *d_y, uint32_t batch, float lr, float *d_loss,
+                        void *stream);
+
 /* Readable name of a status code; last HIP error code seen by the library. */
 const char *tcsg_status_string(int status);
 int tcsg_last_hip_error(void);

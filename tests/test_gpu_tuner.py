@@ -1,0 +1,128 @@
+"""GPU checks of the learned tuner: the HIP network (csrc/tcsg_mlp.hip)
+against its CPU restatement, training on the measured corpus, and the
+top-k ranking against the exhaustive sweep and model-free strategies."""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import pytest
+
+from oracle import mlp
+
+pytestmark = pytest.mark.gpu
+
+IN, H = 33, 128
+
+
+@pytest.fixture(scope="module")
+def tuner():
+    import torch
+    torch.cuda.set_device(0)
+    from threadcoarsenedstencilgenerator_amd import tuner as T
+    assert len(T.FEATURES) == IN
+    return T
+
+
+def _case(seed, B):
+    rng = np.random.default_rng(seed)
+    p = mlp.init_params(IN, H, seed)
+    W1, b1, W2, b2, W3, b3 = mlp.split(p, IN, H)
+    b1[:] = rng.normal(0, 0.3, H).astype(np.float32)
+    b2[:] = rng.normal(0, 0.3, H).astype(np.float32)
+    b3[:] = np.float32(0.1)
+    x = rng.normal(size=(B, IN)).astype(np.float32)
+    y = rng.normal(size=B).astype(np.float32)
+    return p, x, y
+
+
+def test_hip_init_is_the_oracle_init(tuner):
+    m = tuner.CostModel(in_dim=IN, hidden=H, seed=7)
+    print("case: mlp init seed 7", flush=True)
+    assert np.array_equal(m.params().astype(np.float64), mlp.init_params(IN, H, 7))
+
+
+@pytest.mark.parametrize("B", [1, 37, 256])
+def test_hip_forward_and_gradients_match_oracle(tuner, B):
+    print(f"case: mlp forward/gradients B={B}", flush=True)
+    p, x, y = _case(11 + B, B)
+    m = tuner.CostModel(in_dim=IN, hidden=H, max_batch=256)
+    m.set_params(p.astype(np.float32))
+    got = m.forward_raw(x)
+    ref, _ = mlp.forward(p, x, IN, H)
+    assert np.allclose(got, ref, rtol=1e-5, atol=1e-5 * np.abs(ref).max())
+    g, L = m.gradients_raw(x, y)
+    gr, Lr = mlp.gradients(p, x, y, IN, H)
+    # FP32 sums of at most max(B, 128) products against FP64
+    assert np.abs(g - gr).max() <= 1e-4 * np.abs(gr).max() + 1e-7
+    assert L == pytest.approx(Lr, rel=1e-5)
+
+
+def test_hip_adam_steps_match_oracle(tuner):
+    """Two Adam steps: each HIP step against the CPU Adam applied to the HIP
+    gradient at the same parameters (so only the update arithmetic differs)."""
+    print("case: mlp adam 2 steps", flush=True)
+    p, x, y = _case(5, 64)
+    m = tuner.CostModel(in_dim=IN, hidden=H, max_batch=64)
+    m.set_params(p.astype(np.float32))
+    opt = mlp.Adam(p.size, lr=1e-3)
+    cur = p.copy()
+    for _ in range(2):
+        g, _ = m.gradients_raw(x, y)
+        loss_before = m.train_raw(x, y, 1e-3)
+        assert loss_before == pytest.approx(mlp.loss(cur, x, y, IN, H), rel=1e-5)
+        cur = opt.step(cur, g.astype(np.float64))
+        got = m.params().astype(np.float64)
+        assert np.abs(got - cur).max() <= 1e-6
+        cur = got
+
+
+@pytest.fixture(scope="module")
+def corpus(tuner):
+    if not os.path.exists(tuner.DEFAULT_DATA):
+        pytest.skip("no measured corpus in this tree")
+    return tuner.Corpus(tuner.read_rows())
+
+
+def test_training_fits_the_corpus(tuner, corpus):
+    print(f"case: fit corpus rows={len(corpus.rows)}", flush=True)
+    m = tuner.CostModel()
+    losses = m.fit(corpus.X, corpus.y, epochs=20)
+    var = float(np.var(corpus.y))
+    assert losses[-1] < 0.1 * var and losses[-1] < losses[0]
+    pred = m.predict(corpus.X)
+    assert np.corrcoef(pred, corpus.y)[0, 1] > 0.95
+
+
+def test_top8_beats_model_free_tuning(tuner, corpus):
+    """Leave-kernels-out: the model's top-8 must lose less than 8 random
+    shapes and than the 8 shapes best on average, and cost a small part of
+    the exhaustive sweep's timed work."""
+    print("case: 5-fold top-8 evaluation", flush=True)
+    rep = tuner.evaluate(corpus, folds=5, k=8, epochs=40)
+    mr, rr, fr = rep["model_regret"], rep["random_regret"], rep["fixed_regret"]
+    print({k: rep[k] for k in ("model_regret", "random_regret", "fixed_regret", "model_time_frac")}, flush=True)
+    assert mr["mean"] < 0.5 * rr["mean"]
+    assert mr["mean"] <= fr["mean"]
+    assert mr["mean"] < 0.10
+    assert rep["model_time_frac"]["mean"] < 0.15
+
+
+def test_tune_with_the_committed_model_finds_a_near_best_block(tuner):
+    """tune() on a kernel outside the corpus (3D Box Jacobi r=2, 2x2x4,
+    launch bound 256), 256^3: its top-8 best against the full sweep of the
+    same kernel measured here."""
+    if not os.path.exists(tuner.DEFAULT_MODEL):
+        pytest.skip("no trained cost model in this tree")
+    from threadcoarsenedstencilgenerator_amd import manifest
+    from threadcoarsenedstencilgenerator_amd import runtime as rt
+    print("case: tune cfg4 plain 260^3", flush=True)
+    d = manifest.cfg4_desc(shape="plain")
+    k = rt.Kernel(d)
+    size = (260, 260, 260)
+    block, glups, tried = tuner.tune(k, size, k=8)
+    assert len(tried) == 8 and block in tuner.candidates(k, 3)
+    rows = tuner.measure_sweep(d, size)
+    best = max(r["glups"] for r in rows)
+    print(f"tuned {block} {glups:.1f} GLup/s vs sweep best {best:.1f} over {len(rows)} shapes", flush=True)
+    assert glups >= 0.85 * best

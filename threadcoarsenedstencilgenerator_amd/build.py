@@ -36,8 +36,8 @@ def _run(cmd):
 
 
 def build_runtime() -> str:
-    src = os.path.join(PKG, "csrc", "tcsg_runtime.hip")
-    _run([HIPCC, *HIP_FLAGS, "-fPIC", "-shared", "-Wall", f"-I{INCLUDE}", src, "-o", LIB])
+    srcs = [os.path.join(PKG, "csrc", f) for f in ("tcsg_runtime.hip", "tcsg_mlp.hip")]
+    _run([HIPCC, *HIP_FLAGS, "-fPIC", "-shared", "-Wall", f"-I{INCLUDE}", *srcs, "-o", LIB])
     return LIB
 
 

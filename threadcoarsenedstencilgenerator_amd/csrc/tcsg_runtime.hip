@@ -210,6 +210,10 @@ int geometry_compute(const tcsg_desc &d, tcsg_dim3 size, tcsg_geometry &g)
 
 } // namespace
 
+// Lets the other translation units of the library (the tuner's network)
+// record a failed HIP call for tcsg_last_hip_error.
+__attribute__((visibility("hidden"))) void tcsg_note_hip_error(int e) { g_last_hip = e; }
+
 struct tcsg_kernel {
     hipModule_t module = nullptr;
     hipFunction_t fn = nullptr;

@@ -126,6 +126,11 @@ def cfg4_desc(dtype=F64, shape="dpp", batched=False, max_threads=256):
     return d.replace(batched=batched) if batched else d
 
 
+def smoke_desc():
+    """3D Star Variable r=4, coarsening 1x4x16, DPP x exchange (the smoke run's kernel)."""
+    return cfg2_desc(1, 4, 16, shape="dpp", max_threads=256)
+
+
 def default_build_set():
     """Kernels the tests, smoke() and bench.py load.  Tuning candidate sets
     (cfg2_sweep, cfg2_tuned, cfg2_f32c_candidates, ...) are compiled on
@@ -133,6 +138,7 @@ def default_build_set():
     descs = [d for d, _ in parity_cases()] + [d for d, _ in f32c_cases()]
     descs += [cfg1_desc("plain"), cfg1_desc("stream"), cfg1_desc("dppw")]
     descs += [cfg2_desc(1, 1, 1), cfg2_desc(2, 4, 8)]            # reference-shaped kernels
+    descs += [smoke_desc()]
     descs += [cfg4_desc(F64, "plain"), cfg4_desc(F64, "dppw")]
     descs += [cfg4_desc(F64, "dppw", batched=True), cfg4_desc(F32C, "dppw", batched=True),
               cfg4_desc(F32C, "plain")]

@@ -92,6 +92,15 @@ def _lib():
         L.tcsg_batch_info.argtypes = [p, C.POINTER(u64), C.POINTER(u32), C.POINTER(u32)]
         L.tcsg_error_f32_vs_f64.argtypes = [C.POINTER(CGeometry), p, p, C.c_double, C.POINTER(C.c_double)]
         L.tcsg_convert_f64_to_f32.argtypes = [p, p, u64, p]
+        L.tcsg_mlp_num_params.argtypes = [u32, u32]
+        L.tcsg_mlp_num_params.restype = C.c_size_t
+        L.tcsg_mlp_create.argtypes = [C.POINTER(p), u32, u32, u32, u64]
+        L.tcsg_mlp_free.argtypes = [p]
+        L.tcsg_mlp_get_params.argtypes = [p, p, C.c_size_t]
+        L.tcsg_mlp_set_params.argtypes = [p, p, C.c_size_t]
+        L.tcsg_mlp_forward.argtypes = [p, p, u32, p, p]
+        L.tcsg_mlp_gradients.argtypes = [p, p, p, u32, p, p]
+        L.tcsg_mlp_train_step.argtypes = [p, p, p, u32, C.c_float, p, p]
         L.tcsg_status_string.argtypes = [i]
         L.tcsg_status_string.restype = C.c_char_p
         L.tcsg_last_hip_error.argtypes = []

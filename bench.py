@@ -57,7 +57,7 @@ def cpu_baseline(desc, budget_s: float):
     t0 = time.perf_counter()
     oracle.step_f64(desc, g, mask, a, b)
     one = time.perf_counter() - t0
-    steps = max(1, min(20, int(budget_s / max(one, 1e-3)) - 1))
+    steps = max(1, min(400, int(budget_s / max(one, 1e-3)) - 1))
     t0 = time.perf_counter()
     oracle.run_f64(desc, g, mask, b, a, steps)
     dt = time.perf_counter() - t0

@@ -119,6 +119,14 @@ int tcsg_max_block_size(tcsg_kernel *k, int *occupancy_optimal, int *legal_max);
 /* 0 if `block` can be launched with this kernel, else TCSG_ERR_BLOCK_SHAPE. */
 int tcsg_check_block(const tcsg_kernel *k, tcsg_dim3 block);
 
+/* Resources of the loaded kernel (the block-shape cost model's inputs):
+ * VGPRs per lane, private scratch bytes per lane,
+ * and, for a block of `threads` threads, the workgroups the occupancy
+ * calculator lets one CU hold (0 if none fit).  Any output may be NULL.
+ * No reference counterpart: the reference's sweep measures every shape
+ * (hip/benchmark.cpp:221-263) and needs no prediction inputs. */
+int tcsg_kernel_resources(tcsg_kernel *k, uint32_t threads, int *vgprs, int *scratch_bytes, int *workgroups_per_cu);
+
 /* Blocks per axis = ceil(threads / block) (hip/benchmark.cpp:232,301-303). */
 tcsg_dim3 tcsg_grid_blocks(const tcsg_geometry *g, tcsg_dim3 block);
 
@@ -188,6 +196,16 @@ int tcsg_error_f32_vs_f64(const tcsg_geometry *g, const float *a, const double *
 /* FP64 -> FP32 copy of a whole grid (round to nearest).  Asynchronous. */
 int tcsg_convert_f64_to_f32(const double *src, float *dst, uint64_t n, void *stream);
 
+/* HBM bandwidth probe: `reps` launches (after one warm-up) of a plain
+ * 16-byte-per-lane stream kernel over `bytes` bytes; mode 0 copies src to dst,
+ * 1 reads src only, 2 writes dst only; nontemporal selects nontemporal loads
+ * and stores; workgroups 0 = one pass without a grid-stride loop.  Synchronous;
+ * *ms_per_rep gets the HIP-event time per launch.  The copy rate is the
+ * practical ceiling of a one-step stencil (one grid read, one written).
+ * No reference counterpart (the reference reports no bandwidth ceiling). */
+int tcsg_hbm_probe(int mode, int nontemporal, const void *src, void *dst, uint64_t bytes, uint32_t workgroups,
+                   int reps, void *stream, float *ms_per_rep);
+
 /* ---- cost-model network of the learned block-shape tuner (no reference
  * counterpart: the reference measures every block shape, hip/benchmark.cpp:
  * 221-263; the tuner ranks them with this network and measures the top few).
@@ -198,7 +216,7 @@ int tcsg_convert_f64_to_f32(const double *src, float *dst, uint64_t n, void *str
  * b3.  Device pointers; the network owns its parameters and activations. */
 typedef struct tcsg_mlp tcsg_mlp;
 size_t tcsg_mlp_num_params(uint32_t in_dim, uint32_t hidden);
-/* He-uniform weights drawn from `seed` (SplitMix64), zero biases. */
+/* Weights U[-1/sqrt(fan_in), 1/sqrt(fan_in)) drawn from `seed` (SplitMix64), zero biases. */
 int tcsg_mlp_create(tcsg_mlp **out, uint32_t in_dim, uint32_t hidden, uint32_t max_batch, uint64_t seed);
 int tcsg_mlp_free(tcsg_mlp *n);
 /* Host copies of the parameters; set also resets the Adam state. */

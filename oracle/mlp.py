@@ -42,14 +42,14 @@ def _splitmix(state: int):
 
 
 def init_params(in_dim: int, hidden: int, seed: int) -> np.ndarray:
-    """He-uniform weights U[-sqrt(6/fan_in), sqrt(6/fan_in)) from one
+    """Uniform weights U[-1/sqrt(fan_in), 1/sqrt(fan_in)) from one
     SplitMix64 stream (W1, then W2, then W3), zero biases, rounded to float32
     as the HIP network stores them."""
     p = np.zeros(num_params(in_dim, hidden), dtype=np.float64)
     W1, b1, W2, b2, W3, b3 = split(p, in_dim, hidden)
     s = seed & _M64
     for W, fan in ((W1, in_dim), (W2, hidden), (W3, hidden)):
-        lim = np.float32(np.sqrt(np.float32(6.0) / np.float32(fan)))
+        lim = np.float32(np.float32(1.0) / np.sqrt(np.float32(fan)))
         flat = W.reshape(-1)
         for k in range(flat.size):
             s, z = _splitmix(s)

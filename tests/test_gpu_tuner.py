@@ -12,7 +12,7 @@ from oracle import mlp
 
 pytestmark = pytest.mark.gpu
 
-IN, H = 33, 128
+IN, H = 41, 128
 
 
 @pytest.fixture(scope="module")
@@ -78,51 +78,61 @@ def test_hip_adam_steps_match_oracle(tuner):
 
 
 @pytest.fixture(scope="module")
-def corpus(tuner):
-    if not os.path.exists(tuner.DEFAULT_DATA):
+def corpora(tuner):
+    if not (os.path.exists(tuner.TRAIN_DATA) and os.path.exists(tuner.HELDOUT_DATA)):
         pytest.skip("no measured corpus in this tree")
-    return tuner.Corpus(tuner.read_rows())
+    return tuner.Corpus(tuner.read_rows(tuner.TRAIN_DATA)), tuner.Corpus(tuner.read_rows(tuner.HELDOUT_DATA))
 
 
-def test_training_fits_the_corpus(tuner, corpus):
-    print(f"case: fit corpus rows={len(corpus.rows)}", flush=True)
+def test_training_fits_the_corpus(tuner, corpora):
+    train, _ = corpora
+    print(f"case: fit training corpus rows={len(train.rows)}", flush=True)
     m = tuner.CostModel()
-    losses = m.fit(corpus.X, corpus.y, epochs=20)
-    var = float(np.var(corpus.y))
+    losses = m.fit(train.X, train.y, epochs=30)
+    var = float(np.var(train.y))
     assert losses[-1] < 0.1 * var and losses[-1] < losses[0]
-    pred = m.predict(corpus.X)
-    assert np.corrcoef(pred, corpus.y)[0, 1] > 0.95
+    pred = m.predict(train.X)
+    assert np.corrcoef(pred, train.y)[0, 1] > 0.95
 
 
-def test_top8_beats_model_free_tuning(tuner, corpus):
-    """Leave-kernels-out: the model's top-8 must lose less than 8 random
-    shapes and than the 8 shapes best on average, and cost a small part of
-    the exhaustive sweep's timed work."""
-    print("case: 5-fold top-8 evaluation", flush=True)
-    rep = tuner.evaluate(corpus, folds=5, k=8, epochs=40)
+def test_top8_on_heldout_configurations(tuner, corpora):
+    """The network trained by the HIP kernels on the training configurations
+    ranks the block shapes of held-out configurations (kernels it has not
+    seen); timing only its top 8 must land within 5% of the exhaustive
+    sweep's best on average in at most 10% of the sweeps' GPU time, and lose
+    less than the model-free strategies with the same 8 measurements: 8
+    shapes at random, and the 8 shapes best on average over the training
+    configurations."""
+    train, held = corpora
+    print(f"case: top-8 on {len(held.groups)} held-out configurations", flush=True)
+    rep = tuner.evaluate(train, held, k=8, epochs=60)
     mr, rr, fr = rep["model_regret"], rep["random_regret"], rep["fixed_regret"]
-    print({k: rep[k] for k in ("model_regret", "random_regret", "fixed_regret", "model_time_frac")}, flush=True)
+    print({k: rep[k] for k in ("model_regret", "random_regret", "fixed_regret", "model_time_frac",
+                               "model_time_frac_total", "within_5pct")}, flush=True)
+    assert mr["mean"] <= 0.05
+    # GPU time of tuning every held-out configuration (top 8 measured plus the
+    # network's inference) over the time of their exhaustive sweeps
+    assert rep["model_time_frac_total"] <= 0.10
     assert mr["mean"] < 0.5 * rr["mean"]
-    assert mr["mean"] <= fr["mean"]
-    assert mr["mean"] < 0.10
-    assert rep["model_time_frac"]["mean"] < 0.15
+    assert mr["mean"] < fr["mean"]
 
 
 def test_tune_with_the_committed_model_finds_a_near_best_block(tuner):
-    """tune() on a kernel outside the corpus (3D Box Jacobi r=2, 2x2x4,
-    launch bound 256), 256^3: its top-8 best against the full sweep of the
-    same kernel measured here."""
+    """tune() with the committed model on a kernel outside both corpora
+    (3D Box Jacobi r=2, 2x2x4, reference code shape, launch bound 1024) on a
+    2^26-point grid: its top-8 best against the full sweep of the same kernel
+    measured here."""
     if not os.path.exists(tuner.DEFAULT_MODEL):
         pytest.skip("no trained cost model in this tree")
     from threadcoarsenedstencilgenerator_amd import manifest
     from threadcoarsenedstencilgenerator_amd import runtime as rt
-    print("case: tune cfg4 plain 260^3", flush=True)
-    d = manifest.cfg4_desc(shape="plain")
+    d = manifest.cfg4_desc(shape="plain", max_threads=1024)
+    size = (516, 260, 516)
+    print(f"case: tune {d.key()} {size}", flush=True)
     k = rt.Kernel(d)
-    size = (260, 260, 260)
     block, glups, tried = tuner.tune(k, size, k=8)
     assert len(tried) == 8 and block in tuner.candidates(k, 3)
     rows = tuner.measure_sweep(d, size)
     best = max(r["glups"] for r in rows)
     print(f"tuned {block} {glups:.1f} GLup/s vs sweep best {best:.1f} over {len(rows)} shapes", flush=True)
-    assert glups >= 0.85 * best
+    assert glups >= 0.95 * best

@@ -108,12 +108,12 @@ def test_adam_first_step_closed_form():
     assert np.allclose(p1, p0 - 0.01 * g / (np.abs(g) + 1e-8), rtol=1e-12)
 
 
-def test_init_params_he_uniform_bounds():
+def test_init_params_uniform_bounds():
     p = mlp.init_params(33, 128, 7)
     W1, b1, W2, b2, W3, b3 = mlp.split(p, 33, 128)
-    assert np.all(np.abs(W1) <= math.sqrt(6 / 33)) and np.all(np.abs(W2) <= math.sqrt(6 / 128))
+    assert np.all(np.abs(W1) <= 1 / math.sqrt(33)) and np.all(np.abs(W2) <= 1 / math.sqrt(128))
     assert not b1.any() and not b2.any() and not b3.any()
-    assert abs(W2.mean()) < 0.01 and W2.std() == pytest.approx(math.sqrt(6 / 128) / math.sqrt(3), rel=0.02)
+    assert abs(W2.mean()) < 0.01 and W2.std() == pytest.approx(1 / math.sqrt(128) / math.sqrt(3), rel=0.02)
     assert np.array_equal(p, p.astype(np.float32).astype(np.float64))
 
 
@@ -167,18 +167,44 @@ def test_top_k_outcome():
     assert reg == 0.0
 
 
-def test_corpus_file_is_whole_sweeps():
-    """Each (kernel, grid) group of the committed corpus holds one row per
-    launchable sweep shape, with positive throughputs."""
-    if not os.path.exists(tuner.DEFAULT_DATA):
+def test_corpus_files_match_their_configuration_sets():
+    """The committed corpora: the held-out file holds one whole sweep per
+    held-out configuration (every launchable shape once), the training file
+    a sample of shapes per training configuration; no kernel is in both."""
+    if not (os.path.exists(tuner.TRAIN_DATA) and os.path.exists(tuner.HELDOUT_DATA)):
         pytest.skip("no measured corpus in this tree")
-    cp = tuner.Corpus(tuner.read_rows())
-    assert len(cp.groups) >= 60 and np.all(cp.glups > 0) and np.all(np.isfinite(cp.X))
-    for gk, idx in cp.groups.items():
-        d = cp.descs[gk[0]]
-        shapes = [cp.shape[i] for i in idx]
-        assert len(set(shapes)) == len(shapes)
-        allowed = set(tuner.sweep_shapes(d.dimension, d.max_threads))
-        assert set(shapes) <= allowed
-        if d.xext == "dppw" and d.variant == 1:
-            assert all(s[0] % 64 == 0 for s in shapes)
+    train = tuner.Corpus(tuner.read_rows(tuner.TRAIN_DATA))
+    held = tuner.Corpus(tuner.read_rows(tuner.HELDOUT_DATA))
+    assert not set(train.descs) & set(held.descs)
+    for cp, cfgs in ((train, tuner.train_configs()), (held, tuner.heldout_configs())):
+        assert np.all(cp.glups > 0) and np.all(np.isfinite(cp.X))
+        want = {d.key(): tuple(s) + (1,) * (3 - len(s)) for d, s in cfgs}
+        for gk, idx in cp.groups.items():
+            assert want[gk[0]] == gk[1:]
+            d = cp.descs[gk[0]]
+            shapes = [cp.shape[i] for i in idx]
+            assert len(set(shapes)) == len(shapes)
+            assert set(shapes) <= set(tuner.sweep_shapes(d.dimension, d.max_threads))
+            if d.xext == "dppw" and d.variant == 1:
+                assert all(s[0] % 64 == 0 for s in shapes)
+            if cp is train:
+                assert len(shapes) <= tuner.TRAIN_SHAPES_PER_CONFIG
+    assert len(held.groups) >= 0.9 * tuner.HELDOUT_CONFIGS
+
+
+def test_configuration_sampler_stays_in_the_workload_space():
+    """Sampled configurations: 2^26-2^27 interior points, ranges 1-12,
+    coarsening 1-16 per axis, a bounded statement count, distinct kernels,
+    and held-out kernels disjoint from the training kernels."""
+    tr, ho = tuner.train_configs(), tuner.heldout_configs()
+    assert len(tr) == tuner.TRAIN_CONFIGS and len(ho) == tuner.HELDOUT_CONFIGS
+    assert not {d.key() for d, _ in tr} & {d.key() for d, _ in ho}
+    for d, s in tr + ho:
+        pts = math.prod(s[i] - 2 * d.range for i in range(d.dimension))
+        assert 2 ** 26 * 0.98 <= pts <= 2 ** 27 * 1.02, (d, s)
+        assert 1 <= d.range <= 12 and all(c in (1, 2, 4, 8, 16) for c in d.coarsening)
+        assert d.xc * d.yc * d.zc * d.n <= tuner.MAX_TERMS
+        for a in range(d.dimension):
+            assert (s[a] - 2 * d.range) % d.coarsening[a] == 0
+    dims = {d.dimension for d, _ in ho}
+    assert dims == {2, 3} and {d.variant for d, _ in ho} == {0, 1}

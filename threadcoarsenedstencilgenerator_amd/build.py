@@ -36,7 +36,7 @@ def _run(cmd):
 
 
 def build_runtime() -> str:
-    srcs = [os.path.join(PKG, "csrc", f) for f in ("tcsg_runtime.hip", "tcsg_mlp.hip")]
+    srcs = [os.path.join(PKG, "csrc", f) for f in ("tcsg_runtime.hip", "tcsg_mlp.hip", "tcsg_hbm_probe.hip")]
     _run([HIPCC, *HIP_FLAGS, "-fPIC", "-shared", "-Wall", f"-I{INCLUDE}", *srcs, "-o", LIB])
     return LIB
 

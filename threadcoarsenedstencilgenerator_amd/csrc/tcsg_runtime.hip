@@ -290,6 +290,19 @@ int tcsg_kernel_load(tcsg_kernel **out, const char *path, const tcsg_desc *expec
     e = hipFuncGetAttribute(&v, HIP_FUNC_ATTRIBUTE_MAX_THREADS_PER_BLOCK, k->fn);
     if (e != hipSuccess) { g_last_hip = (int)e; return fail(TCSG_ERR_HIP); }
     k->legal_max = std::min<int>(v, (int)q[15]);
+    // The attribute bounds the block by the launch bound only; a kernel whose
+    // registers do not fit that many waves on one CU cannot be launched with
+    // such a block, so lower the bound until the occupancy calculator places
+    // at least one workgroup.
+    k->legal_max -= k->legal_max % 64;
+    while (k->legal_max >= 64) {
+        int nb = 0;
+        e = hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k->fn, k->legal_max, 0);
+        if (e != hipSuccess) { g_last_hip = (int)e; return fail(TCSG_ERR_HIP); }
+        if (nb > 0) break;
+        k->legal_max -= 64;
+    }
+    if (k->legal_max < 64) return fail(TCSG_ERR_BLOCK_SHAPE);
     k->xext_mode = d.variant == TCSG_STREAM ? (int)q[16] : 0;
     if (k->xext_mode > 2) return fail(TCSG_ERR_MISMATCH);
     *out = k;
@@ -338,6 +351,21 @@ int tcsg_max_block_size(tcsg_kernel *k, int *occ, int *legal)
         *occ = block;
     }
     if (legal) *legal = k->legal_max;
+    return TCSG_OK;
+}
+
+int tcsg_kernel_resources(tcsg_kernel *k, uint32_t threads, int *vgprs, int *scratch_bytes, int *workgroups_per_cu)
+{
+    if (!k) return TCSG_ERR_ARG;
+    if (vgprs) TCSG_HIP(hipFuncGetAttribute(vgprs, HIP_FUNC_ATTRIBUTE_NUM_REGS, k->fn));
+    if (scratch_bytes) TCSG_HIP(hipFuncGetAttribute(scratch_bytes, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, k->fn));
+    if (workgroups_per_cu) {
+        *workgroups_per_cu = 0;
+        if (threads == 0 || threads > (uint32_t)k->legal_max) return TCSG_OK;
+        int n = 0;
+        TCSG_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&n, k->fn, (int)threads, 0));
+        *workgroups_per_cu = n;
+    }
     return TCSG_OK;
 }
 
@@ -424,8 +452,13 @@ static int launch_one(tcsg_kernel *k, const tcsg_geometry *g, tcsg_dim3 block, t
     a.in_bytes = a.out_bytes = (uint32_t)std::min<uint64_t>(bytes, 0xFFFFFFFFull);
     size_t sz = sizeof(a);
     void *extra[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, &a, HIP_LAUNCH_PARAM_BUFFER_SIZE, &sz, HIP_LAUNCH_PARAM_END};
-    TCSG_HIP(hipModuleLaunchKernel(k->fn, blocks.x, blocks.y, blocks.z, block.x, block.y, block.z, 0, s, nullptr,
-                                   extra));
+    const hipError_t e = hipModuleLaunchKernel(k->fn, blocks.x, blocks.y, blocks.z, block.x, block.y, block.z, 0, s,
+                                               nullptr, extra);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();  // a rejected launch must not surface in the caller's next HIP call
+        g_last_hip = (int)e;
+        return TCSG_ERR_HIP;
+    }
     return TCSG_OK;
 }
 
@@ -440,6 +473,8 @@ static int precheck(tcsg_kernel *k, const tcsg_geometry *g, tcsg_dim3 block, tcs
     if ((uint64_t)blocks.x * block.x < g->threads.x || (uint64_t)blocks.y * block.y < g->threads.y ||
         (uint64_t)blocks.z * block.z < g->threads.z)
         return TCSG_ERR_ARG;  // grid would not cover the interior
+    // the launch API's grid limits: 2^31-1 blocks in x, 65535 in y and z
+    if (blocks.x > 0x7FFFFFFFu || blocks.y > 65535u || blocks.z > 65535u) return TCSG_ERR_BLOCK_SHAPE;
     if (k->desc.coeff == TCSG_VARIABLE && !k->mask_set) return TCSG_ERR_NO_MASK;
     return TCSG_OK;
 }

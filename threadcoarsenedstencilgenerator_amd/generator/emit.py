@@ -162,6 +162,7 @@ class _Emitter:
           f"alignment {['None', 'Read', 'Write'][d.alignment]}/{d.cl_size} "
           f"{'f32c' if self.f32 else 'f64'} {'stream' if d.variant == STREAM else 'plain'} "
           f"xext={d.xext} xcd_remap={int(d.xcd_remap)} nt={int(d.nt_store)}"
+          + (" nt_load=1" if d.nt_load else "")
           + (" (CPU emulation build)" if cpu else ""))
         if cpu:
             w('#include "cpu_emu.h"')
@@ -235,8 +236,9 @@ class _Emitter:
                 w("__device__ __forceinline__ __amdgpu_buffer_rsrc_t tcsg_rsrc(const void* p, uint32_t n) {")
                 w("  return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)n, 0x00020000);")
                 w("}")
+                laux = 2 if d.nt_load else 0
                 w(f"__device__ __forceinline__ {T} tcsg_ld(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff) {{")
-                w(f"  return __builtin_bit_cast({T}, __builtin_amdgcn_raw_buffer_load_b{bits}(r, off, soff, 0));")
+                w(f"  return __builtin_bit_cast({T}, __builtin_amdgcn_raw_buffer_load_b{bits}(r, off, soff, {laux}));")
                 w("}")
                 aux = 2 if d.nt_store else 0
                 w(f"__device__ __forceinline__ void tcsg_st(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff, {T} v) {{")

@@ -7,17 +7,50 @@ counter pass measures both.
 
     rocprofv3 --pmc FETCH_SIZE -d OUT -o pmc --output-format csv -- \
         python -m threadcoarsenedstencilgenerator_amd.profile_run [--dtype f64|f32c] [--steps 3]
+
+With --ceiling it instead times the HBM stream probes (copy, read-only,
+write-only; plain and nontemporal; one pass and grid-stride launches) over
+2 GiB buffers (past every cache level, about twice the 512^3 workload's grid)
+and prints one JSON line per probe: the practical bandwidth ceiling the
+stencil kernels are compared with.
 """
 from __future__ import annotations
 
 import argparse
 
 
+def ceiling(reps: int = 20):
+    import json
+
+    import torch
+
+    from . import runtime as rt
+    torch.cuda.set_device(0)
+    nbytes = 2 * 2 ** 30 - (2 * 2 ** 30) % (1 << 20)   # 2 GiB per buffer: past every cache level
+    a = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    b = torch.zeros_like(a)
+    out = []
+    for mode in ("copy", "read", "write"):
+        moved = 2 * nbytes if mode == "copy" else nbytes
+        for nt in (False, True):
+            for wg in (0, 256 * 4, 256 * 8, 256 * 16, 256 * 32):
+                ms = rt.hbm_probe(mode, a.data_ptr(), b.data_ptr(), nbytes, nt, wg, reps)
+                line = dict(probe=mode, nontemporal=nt, workgroups=wg or "one pass", bytes_per_launch=moved,
+                            ms=round(ms, 4), tbs=round(moved / ms / 1e9, 3))
+                out.append(line)
+                print(json.dumps(line), flush=True)
+    best = {m: max(x["tbs"] for x in out if x["probe"] == m) for m in ("copy", "read", "write")}
+    print(json.dumps({"hbm_ceiling_tbs": best, "peak_tbs": 8.0}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--dtype", default="f64", choices=("f64", "f32c"))
+    ap.add_argument("--ceiling", action="store_true")
     args = ap.parse_args()
+    if args.ceiling:
+        return ceiling()
     import torch
 
     from . import runtime as rt

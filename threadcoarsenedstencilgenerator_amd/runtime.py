@@ -75,6 +75,7 @@ def _lib():
         L.tcsg_set_mask.argtypes = [p, p, u32]
         L.tcsg_max_block_size.argtypes = [p, C.POINTER(i), C.POINTER(i)]
         L.tcsg_check_block.argtypes = [p, Dim3]
+        L.tcsg_kernel_resources.argtypes = [p, u32, C.POINTER(i), C.POINTER(i), C.POINTER(i)]
         L.tcsg_grid_blocks.argtypes = [C.POINTER(CGeometry), Dim3]
         L.tcsg_grid_blocks.restype = Dim3
         L.tcsg_launch_init.argtypes = [p, C.POINTER(CGeometry), i, u64, p, p]
@@ -92,6 +93,7 @@ def _lib():
         L.tcsg_batch_info.argtypes = [p, C.POINTER(u64), C.POINTER(u32), C.POINTER(u32)]
         L.tcsg_error_f32_vs_f64.argtypes = [C.POINTER(CGeometry), p, p, C.c_double, C.POINTER(C.c_double)]
         L.tcsg_convert_f64_to_f32.argtypes = [p, p, u64, p]
+        L.tcsg_hbm_probe.argtypes = [i, i, p, p, u64, u32, i, p, C.POINTER(C.c_float)]
         L.tcsg_mlp_num_params.argtypes = [u32, u32]
         L.tcsg_mlp_num_params.restype = C.c_size_t
         L.tcsg_mlp_create.argtypes = [C.POINTER(p), u32, u32, u32, u64]
@@ -202,6 +204,13 @@ class Kernel:
     def block_ok(self, block) -> bool:
         return _lib().tcsg_check_block(self._h, dim3(block)) == OK
 
+    def resources(self, threads: int = 0):
+        """(VGPRs per lane, scratch bytes per lane, workgroups of `threads`
+        threads one CU holds)."""
+        v, s, w = C.c_int(), C.c_int(), C.c_int()
+        _check(_lib().tcsg_kernel_resources(self._h, threads, C.byref(v), C.byref(s), C.byref(w)), "resources")
+        return v.value, s.value, w.value
+
     def geometry(self, size) -> CGeometry:
         return geometry(self.desc, size)
 
@@ -245,6 +254,15 @@ def error_f32_vs_f64(g: CGeometry, a_ptr: int, b_ptr: int, floor: float = 1e-3):
 
 def convert_f64_to_f32(src_ptr: int, dst_ptr: int, n: int, stream=None):
     _check(_lib().tcsg_convert_f64_to_f32(src_ptr, dst_ptr, n, _stream(stream)), "convert")
+
+
+def hbm_probe(mode: str, src: int, dst: int, nbytes: int, nontemporal: bool = False, workgroups: int = 0,
+              reps: int = 10, stream=None) -> float:
+    """Milliseconds per launch of the copy / read / write bandwidth probe."""
+    ms = C.c_float()
+    _check(_lib().tcsg_hbm_probe({"copy": 0, "read": 1, "write": 2}[mode], int(nontemporal), src, dst, nbytes,
+                                 workgroups, reps, _stream(stream), C.byref(ms)), "hbm_probe")
+    return ms.value
 
 
 class Grids:

@@ -36,7 +36,7 @@ class StencilDesc:
     dimension/neighbourhood/orientation/coeff/range/coarsening/alignment/
     cl_size mirror the reference generator argv.  dtype, variant and the
     tuning knobs (max_threads, xext, xcd_remap, nt_store, comp_group,
-    prefetch, waves_per_eu, zfirst, batched) have no reference counterpart: they select the
+    prefetch, waves_per_eu, zfirst, batched, nt_load) have no reference counterpart: they select the
     MI355X code shape.
     """
 
@@ -65,6 +65,7 @@ class StencilDesc:
     #                          share halo planes) run together on one XCD
     batched: bool = False    # STREAM: also emit tcsg_update_batched, a persistent kernel over a
     #                          tile table spanning many grids
+    nt_load: bool = False    # STREAM: nontemporal hint on the input loads
 
     def __post_init__(self):
         if self.dimension not in (2, 3):
@@ -125,6 +126,9 @@ class StencilDesc:
     def key(self) -> str:
         """Stable short name used for code-object files."""
         d = dataclasses.asdict(self)
+        for f, default in _LATER_FIELDS.items():   # keep the names of kernels that predate these knobs
+            if d[f] == default:
+                del d[f]
         h = hashlib.sha1(repr(sorted(d.items())).encode()).hexdigest()[:10]
         nb = "box" if self.neighbourhood == BOX else "star"
         return (f"{self.dimension}d_{nb}_r{self.range}_{_inv(COEFF_NAMES)[self.coeff].lower()}"
@@ -140,11 +144,17 @@ class StencilDesc:
         return [0x54435347, 1, self.dimension, self.neighbourhood, self.orientation,
                 self.coeff, self.range, self.xc, self.yc, self.zc, self.alignment,
                 self.cl_size, self.dtype, self.variant, self.n, self.max_threads,
-                {"load": 0, "dpp": 1, "dppw": 2}[self.xext], int(self.xcd_remap), int(self.nt_store),
+                {"load": 0, "dpp": 1, "dppw": 2}[self.xext], int(self.xcd_remap),
+                int(self.nt_store) | int(self.nt_load) << 1,
                 self.comp_group, self.prefetch, self.waves_per_eu, int(self.zfirst), int(self.batched)]
 
     def replace(self, **kw) -> "StencilDesc":
         return dataclasses.replace(self, **kw)
+
+
+# knobs added after kernel names were first recorded (measured corpora refer to
+# kernels by name): left out of the name's hash while at their default
+_LATER_FIELDS = {"nt_load": False}
 
 
 def mask_index(desc: StencilDesc, dx: int, dy: int, dz: int = 0):

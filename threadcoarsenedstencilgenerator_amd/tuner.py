@@ -4,19 +4,29 @@ The reference finds a kernel's block shape by measuring every legal one: the
 harness sweeps total thread counts 64, 128, ... up to the launch bound and
 every factorisation of each into (x, y[, z]) (hip/benchmark.cpp:221-263 with
 hip/primes.cpp).  This module predicts each candidate's throughput with a
-small cost model, measures only the top k predictions and keeps the best.
+small cost model, measures only the top k predictions and keeps the best;
+without a trained model it falls back to the exhaustive sweep.
 
 The cost model is a 3-layer MLP (128 hidden units, ReLU, FP32) trained with
 Adam by the HIP kernels of csrc/tcsg_mlp.hip.  Its inputs are the stencil's
-parameters, the grid shape, the block shape and the coarsening
-(``features``); its target is log GLup/s.  The training corpus is a set of
-full sweeps measured on MI355X (``tuner_data/``), gathered by
-``collect`` over the kernels of ``corpus()``.
+parameters, the grid shape, the block shape, the coarsening and the loaded
+kernel's resources (``features``); its target is log GLup/s.
 
-    python -m threadcoarsenedstencilgenerator_amd.tuner build            # compile the corpus kernels
-    python -m threadcoarsenedstencilgenerator_amd.tuner collect OUT.csv  # measure the sweeps (GPU)
-    python -m threadcoarsenedstencilgenerator_amd.tuner evaluate DATA    # k-fold regret report (GPU)
-    python -m threadcoarsenedstencilgenerator_amd.tuner fit DATA MODEL   # train on all of DATA (GPU)
+Data (tuner_data/, measured on MI355X by ``collect``): configurations are
+drawn by ``sample_config`` from the tuning workload (2D/3D, Box/Star,
+Jacobi/Variable, range 1-12, coarsening 1-16 per axis, 2^26-2^27 interior
+points, launch bounds up to 1024 threads).  The training set times 16
+random launchable block shapes of each of 250 configurations (about 3800
+samples; its first 125 configurations, about 1900 samples, form the smaller
+training set the evaluation also reports); the held-out set times every
+launchable shape of 200 further configurations whose kernels the training
+set does not contain.
+
+    python -m threadcoarsenedstencilgenerator_amd.tuner build               # compile both sets' kernels
+    python -m threadcoarsenedstencilgenerator_amd.tuner collect train       # measure (GPU), resumable
+    python -m threadcoarsenedstencilgenerator_amd.tuner collect heldout
+    python -m threadcoarsenedstencilgenerator_amd.tuner evaluate --json R   # top-k regret on held-out (GPU)
+    python -m threadcoarsenedstencilgenerator_amd.tuner fit                 # train the committed model (GPU)
 """
 from __future__ import annotations
 
@@ -36,7 +46,9 @@ from .spec import (ALIGN_NONE, ALIGN_READ, ALIGN_WRITE, BOX, COLUMN, F32C, F64, 
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 DATA_DIR = os.path.join(HERE, "tuner_data")
-DEFAULT_DATA = os.path.join(DATA_DIR, "sweeps_mi355x.csv.gz")
+TRAIN_DATA = os.path.join(DATA_DIR, "train_mi355x.csv.gz")
+HELDOUT_DATA = os.path.join(DATA_DIR, "heldout_mi355x.csv.gz")
+DATA = {"train": TRAIN_DATA, "heldout": HELDOUT_DATA}
 DEFAULT_MODEL = os.path.join(DATA_DIR, "cost_model_mi355x.npz")
 WARP = 64
 
@@ -86,12 +98,16 @@ FEATURES = ("is3d", "star", "variable", "column", "range", "log2_points", "log2_
             "log2_nx", "log2_ny", "log2_nz", "log2_bx", "log2_by", "log2_bz", "log2_threads", "log2_tile_x",
             "log2_tile_y", "log2_tile_z", "log2_workgroups", "log2_cover", "f32c", "stream", "xext_load",
             "xext_dpp", "xext_dppw", "log2_max_threads", "zfirst", "prefetch", "waves_per_eu", "bytes_per_point",
-            "log2_tiles_x", "log2_tiles_yz")
+            "log2_tiles_x", "log2_tiles_yz", "log2_vgprs", "log2_scratch", "wg_per_cu", "waves_per_simd",
+            "wave_rounds", "last_round_fill", "log2_terms", "row_bytes_per_wave")
+CUS, SIMDS_PER_CU = 256, 4
 
 
-def features(desc: StencilDesc, size, block) -> np.ndarray:
+def features(desc: StencilDesc, size, block, res=(0, 0, 0)) -> np.ndarray:
     """Input vector of the cost model for one (kernel, grid, block) triple.
-    size is the grid extent including the halo, as the harness takes it."""
+    size is the grid extent including the halo, as the harness takes it;
+    res = (VGPRs per lane, scratch bytes per lane, workgroups per CU at this
+    block size) of the loaded kernel (runtime Kernel.resources)."""
     r = desc.range
     dim = desc.dimension
     size = tuple(size) + (1,) * (3 - len(tuple(size)))
@@ -105,20 +121,38 @@ def features(desc: StencilDesc, size, block) -> np.ndarray:
         cover *= tiles[i] * tile[i] / interior[i]
     l2 = math.log2
     stream = desc.variant == STREAM
+    threads = bl[0] * bl[1] * bl[2]
+    vgprs, scratch, wgpc = (int(v) for v in res)
+    wg = tiles[0] * tiles[1] * tiles[2]
+    waves_wg = (threads + 63) // 64
+    per_round = CUS * max(1, wgpc)
+    rounds = math.ceil(wg / per_round)
+    es = 4 if desc.dtype == F32C else 8
     f = [
         float(dim == 3), float(desc.neighbourhood == STAR), float(desc.coeff == VARIABLE),
         float(desc.orientation == COLUMN), float(r), l2(desc.n), l2(co[0]), l2(co[1]), l2(co[2]),
         l2(interior[0]), l2(interior[1]), l2(interior[2]), l2(bl[0]), l2(bl[1]), l2(bl[2]),
-        l2(bl[0] * bl[1] * bl[2]), l2(tile[0]), l2(tile[1]), l2(tile[2]), l2(tiles[0] * tiles[1] * tiles[2]),
+        l2(threads), l2(tile[0]), l2(tile[1]), l2(tile[2]), l2(wg),
         l2(cover), float(desc.dtype == F32C), float(stream), float(stream and desc.xext == "load"),
         float(stream and desc.xext == "dpp"), float(stream and desc.xext == "dppw"), l2(desc.max_threads),
         float(desc.zfirst), float(desc.prefetch if stream else 0), float(desc.waves_per_eu),
-        4.0 if desc.dtype == F32C else 8.0, l2(tiles[0]), l2(tiles[1] * tiles[2]),
+        float(es), l2(tiles[0]), l2(tiles[1] * tiles[2]),
+        l2(max(1, vgprs)), l2(1 + scratch), float(wgpc), wgpc * waves_wg / SIMDS_PER_CU,
+        l2(rounds), wg / (rounds * per_round), l2(co[0] * co[1] * co[2] * desc.n),
+        # bytes one wave's row segment spans in x (coalescing)
+        l2(min(bl[0], 64) * co[0] * es),
     ]
     return np.asarray(f, dtype=np.float64)
 
 
 # -------------------------------------------------------------------- corpus
+TUNE_SHAPES = ("plain", "stream", "dpp", "dppw")
+MAX_TERMS = 4096          # outputs per thread x stencil size of a sampled kernel
+TRAIN_CONFIGS, HELDOUT_CONFIGS = 250, 200
+TRAIN_SHAPES_PER_CONFIG = 16
+TRAIN_SEED, HELDOUT_SEED = 11, 12
+
+
 def _desc(dim, nb, coeff, r, co, shape, mt, align=ALIGN_NONE, cl=0, dtype=F64, orient=ROW):
     kw = dict(variant=PLAIN)
     if shape != "plain":
@@ -129,59 +163,79 @@ def _desc(dim, nb, coeff, r, co, shape, mt, align=ALIGN_NONE, cl=0, dtype=F64, o
                        max_threads=mt, **kw)
 
 
-def corpus():
-    """(kernel, grid sizes incl. halo) pairs whose full sweeps form the
-    training corpus: 3D grids of 128^3, 256^3 and 512^3 interior points, 2D
-    grids of 1024^2, 4096^2 and 16384^2, over stencil shapes, coarsenings
-    and code shapes."""
-    W, R = ALIGN_WRITE, ALIGN_READ
-    k3 = [
-        _desc(3, STAR, VARIABLE, 4, (1, 1, 1), "plain", 512, W, 16),
-        _desc(3, STAR, VARIABLE, 4, (2, 4, 8), "plain", 512, W, 16),
-        _desc(3, STAR, VARIABLE, 4, (1, 2, 4), "stream", 256, W, 16),
-        _desc(3, STAR, VARIABLE, 4, (1, 4, 16), "dpp", 256, W, 16),
-        _desc(3, STAR, VARIABLE, 4, (2, 4, 32), "dppw", 512, W, 16),
-        _desc(3, STAR, JACOBI, 1, (1, 1, 1), "plain", 512),
-        _desc(3, STAR, JACOBI, 1, (1, 2, 8), "dpp", 256),
-        _desc(3, STAR, JACOBI, 1, (2, 2, 16), "dppw", 512),
-        _desc(3, STAR, VARIABLE, 2, (1, 2, 4), "plain", 512, W, 16),
-        _desc(3, STAR, VARIABLE, 2, (1, 4, 16), "dppw", 512, W, 16),
-        _desc(3, STAR, VARIABLE, 3, (1, 2, 8), "dpp", 256, R, 8, orient=COLUMN),
-        _desc(3, BOX, JACOBI, 2, (2, 2, 4), "plain", 512),
-        _desc(3, BOX, JACOBI, 2, (2, 2, 4), "dpp", 256),
-        _desc(3, BOX, JACOBI, 2, (2, 2, 4), "dppw", 512),
-        _desc(3, BOX, VARIABLE, 1, (1, 1, 1), "plain", 512, W, 16),
-        _desc(3, BOX, VARIABLE, 1, (1, 2, 8), "dpp", 256, W, 16),
-        _desc(3, BOX, VARIABLE, 1, (2, 2, 8), "dppw", 512, W, 16),
-        _desc(3, STAR, VARIABLE, 4, (2, 4, 32), "dppw", 512, W, 16, dtype=F32C),
-        _desc(3, BOX, JACOBI, 2, (2, 2, 4), "plain", 512, dtype=F32C),
-    ]
-    k2 = [
-        _desc(2, BOX, JACOBI, 1, (1, 1, 1), "plain", 512),
-        _desc(2, BOX, JACOBI, 1, (1, 1, 1), "stream", 256),
-        _desc(2, BOX, JACOBI, 1, (1, 1, 1), "dppw", 512),
-        _desc(2, BOX, JACOBI, 1, (2, 4, 1), "dppw", 512),
-        _desc(2, BOX, VARIABLE, 8, (1, 1, 1), "plain", 512, R, 16),
-        _desc(2, BOX, VARIABLE, 8, (2, 2, 1), "dpp", 256, R, 16),
-        _desc(2, BOX, VARIABLE, 8, (1, 4, 1), "dppw", 512, R, 16),
-        _desc(2, STAR, VARIABLE, 4, (1, 4, 1), "dpp", 256, W, 16),
-        _desc(2, STAR, VARIABLE, 4, (2, 8, 1), "dppw", 512, W, 16),
-        _desc(2, STAR, JACOBI, 2, (1, 8, 1), "stream", 256),
-        _desc(2, BOX, VARIABLE, 8, (2, 2, 1), "dppw", 512, R, 16, dtype=F32C),
-    ]
+def sample_config(rng):
+    """One stencil configuration of the tuning workload: 2D/3D, Box/Star,
+    Jacobi/Variable, Row/Column, range 1-12 (3D Box up to 4, so one step
+    stays within a few milliseconds), coarsening 1-16 per axis (powers of
+    two, at most MAX_TERMS accumulate statements per thread), code shape,
+    launch bound 256-1024, alignment, FP64 or FP32-compensated storage, and
+    a grid of 2^26-2^27 interior points with a random aspect ratio.
+    Returns (desc, grid size incl. halo)."""
+    while True:
+        dim = int(rng.integers(2, 4))
+        nb = int(rng.integers(0, 2))
+        coeff = int(rng.integers(0, 2))
+        orient = int(rng.integers(0, 2))
+        r = int(rng.integers(1, (4 if dim == 3 and nb == BOX else 12) + 1))
+        shape = TUNE_SHAPES[int(rng.choice(4, p=[0.4, 0.2, 0.2, 0.2]))]
+        dtype = F32C if rng.random() < 0.2 else F64
+        co = [2 ** int(rng.integers(0, 5)) for _ in range(3)]
+        if dim == 2:
+            co[2] = 1
+        if dtype == F32C:
+            co[0] = min(co[0], 2)
+        n = (2 * r + 1) ** dim if nb == BOX else 2 * dim * r + 1
+        if co[0] * co[1] * co[2] * n > MAX_TERMS:
+            continue
+        mt = int(rng.choice([256, 512, 1024]))
+        align = int(rng.integers(0, 3))
+        cl = (16 if dtype == F64 else 32) if align != ALIGN_NONE else 0
+        e = rng.uniform(26.0, 27.0)
+        lo = 6.0 if dim == 3 else 9.0
+        frac = rng.dirichlet([3.0] * dim)
+        interior = []
+        for a in range(dim):
+            length = 2.0 ** (lo + frac[a] * (e - dim * lo))
+            interior.append(max(co[a], int(round(length / co[a])) * co[a]))
+        if dim == 3 and interior[2] < 2 * r:
+            continue
+        d = _desc(dim, nb, coeff, r, co, shape, mt, align, cl, dtype, orient)
+        return d, tuple(v + 2 * r for v in interior)
+
+
+def sample_configs(n: int, seed: int, exclude=()):
+    """n configurations with distinct kernels, none of them in `exclude`
+    (kernel keys)."""
+    rng = np.random.default_rng(seed)
+    seen = set(exclude)
     out = []
-    for d in k3:
-        out.append((d, [(n + 2 * d.range,) * 3 for n in (128, 256, 512)]))
-    for d in k2:
-        out.append((d, [(n + 2 * d.range,) * 2 for n in (1024, 4096, 16384)]))
+    while len(out) < n:
+        d, size = sample_config(rng)
+        if d.key() in seen:
+            continue
+        seen.add(d.key())
+        out.append((d, size))
     return out
+
+
+def train_configs():
+    return sample_configs(TRAIN_CONFIGS, TRAIN_SEED)
+
+
+def heldout_configs():
+    """Held-out configurations: kernels that no training configuration uses."""
+    return sample_configs(HELDOUT_CONFIGS, HELDOUT_SEED, exclude={d.key() for d, _ in train_configs()})
+
+
+def config_set(name: str):
+    return {"train": train_configs, "heldout": heldout_configs}[name]()
 
 
 # -------------------------------------------------------------- measurements
 COLUMNS = ("key", "dimension", "neighbourhood", "orientation", "coeff", "range", "xc", "yc", "zc", "alignment",
            "cl_size", "dtype", "variant", "max_threads", "xext", "xcd_remap", "nt_store", "comp_group",
-           "prefetch", "waves_per_eu", "zfirst", "sx", "sy", "sz", "bx", "by", "bz", "steps", "ms_per_step",
-           "glups")
+           "prefetch", "waves_per_eu", "zfirst", "sx", "sy", "sz", "bx", "by", "bz", "vgprs", "scratch",
+           "wg_per_cu", "steps", "ms_per_step", "glups")
 _DESC_INT = ("dimension", "neighbourhood", "orientation", "coeff", "range", "xc", "yc", "zc", "alignment",
              "cl_size", "dtype", "variant", "max_threads", "comp_group", "prefetch", "waves_per_eu")
 _DESC_BOOL = ("xcd_remap", "nt_store", "zfirst")
@@ -193,42 +247,52 @@ def row_desc(row) -> StencilDesc:
     return StencilDesc(xext=row["xext"], **kw)
 
 
-def _steps_for(points: int) -> int:
-    return int(min(40, max(2, round(2e8 / points))))
+SLOW_MS = 5.0   # a shape whose first step takes longer is timed by that step alone
 
 
-def measure_sweep(desc: StencilDesc, size, shapes=None):
-    """Time every candidate block shape of one kernel on one grid (HIP
-    events, back-to-back steps).  Returns rows for the corpus."""
+def measure_sweep(desc: StencilDesc, size, shapes=None, mask_seed: int = 2, sample=None):
+    """Time candidate block shapes of one kernel on one grid (HIP events on
+    back-to-back steps): per shape one step, which also warms the shape up,
+    then two timed steps unless the first took longer than SLOW_MS.  shapes
+    defaults to every launchable shape of the sweep; sample=(n, seed) times
+    n of them drawn at random instead.  Returns corpus rows."""
     import torch
 
     from . import runtime as rt
     k = rt.Kernel(desc)
     g = k.geometry(size)
     if desc.coeff == VARIABLE:
-        k.set_mask(rt.mask_seeded(2, desc.n))
+        k.set_mask(rt.mask_seeded(mask_seed, desc.n))
     grids = rt.Grids(g, k.dtype)
     for i in range(2):
         k.init(g, grids.ptr(i), rt.INIT_U01, 1 + i)
     torch.cuda.synchronize()
     pts = g.interior_points
-    steps = _steps_for(pts)
     rows = []
     shapes = shapes if shapes is not None else candidates(k, desc.dimension)
+    if sample is not None and len(shapes) > sample[0]:
+        pick = np.random.default_rng(sample[1]).choice(len(shapes), size=sample[0], replace=False)
+        shapes = [shapes[i] for i in sorted(pick)]
     base = {f: getattr(desc, f) for f in _DESC_INT + _DESC_BOOL}
     base.update(key=desc.key(), xext=desc.xext)
     sz = tuple(size) + (1,) * (3 - len(size))
+    occ = {}
     for s in shapes:
         blocks = rt.grid_blocks(g, s)
+        t = s[0] * s[1] * s[2]
+        if t not in occ:
+            occ[t] = k.resources(t)
+        vg, sc, wgpc = occ[t]
         try:
             one = k.time_steps(g, s, blocks, grids.ptr(0), grids.ptr(1), 0, 1)
         except rt.TcsgError as e:   # a shape the launch rejects is left out, as the harness prints and moves on
             print(f"    block {s}: {e}", flush=True)
             continue
-        n = steps if one < 20.0 else 1
-        ms = one if n == 1 else k.time_steps(g, s, blocks, grids.ptr(0), grids.ptr(1), 1, n) / n
-        rows.append(dict(base, sx=sz[0], sy=sz[1], sz=sz[2], bx=s[0], by=s[1], bz=s[2], steps=n,
-                         ms_per_step=round(ms, 6), glups=round(pts / (ms * 1e-3) / 1e9, 4)))
+        n = 1 if one > SLOW_MS else 2
+        ms = one if n == 1 else k.time_steps(g, s, blocks, grids.ptr(0), grids.ptr(1), 0, n) / n
+        rows.append(dict(base, sx=sz[0], sy=sz[1], sz=sz[2], bx=s[0], by=s[1], bz=s[2], vgprs=vg, scratch=sc,
+                         wg_per_cu=wgpc, steps=n, ms_per_step=round(ms, 6),
+                         glups=round(pts / (ms * 1e-3) / 1e9, 4)))
     del grids
     k.close()
     torch.cuda.empty_cache()
@@ -246,7 +310,7 @@ def write_rows(path: str, rows, append=False):
             w.writerow({c: (int(v) if isinstance(v, bool) else v) for c, v in ((c, r[c]) for c in COLUMNS)})
 
 
-def read_rows(path: str = DEFAULT_DATA):
+def read_rows(path: str):
     opener = gzip.open if path.endswith(".gz") else open
     with opener(path, "rt", newline="") as f:
         return list(csv.DictReader(f))
@@ -265,10 +329,12 @@ class Corpus:
             if r["key"] not in descs:
                 descs[r["key"]] = row_desc(r)
         self.descs = descs
-        self.X = np.stack([features(descs[r["key"]], self._size(r), (int(r["bx"]), int(r["by"]), int(r["bz"])))
+        self.X = np.stack([features(descs[r["key"]], self._size(r), (int(r["bx"]), int(r["by"]), int(r["bz"])),
+                                    (int(r["vgprs"]), int(r["scratch"]), int(r["wg_per_cu"])))
                            for r in rows]) if rows else np.zeros((0, len(FEATURES)))
         self.glups = np.array([float(r["glups"]) for r in rows])
         self.ms = np.array([float(r["ms_per_step"]) for r in rows])
+        self.steps = np.array([int(r["steps"]) for r in rows], dtype=np.int64)
         self.shape = [(int(r["bx"]), int(r["by"]), int(r["bz"])) for r in rows]
         self.y = np.log(self.glups)
 
@@ -370,7 +436,7 @@ class CostModel:
         torch.cuda.synchronize()
         return float(ld.item())
 
-    def fit(self, X, y, epochs: int = 60, batch: int = 256, lr: float = 2e-3, seed: int = 0):
+    def fit(self, X, y, epochs: int = 60, batch: int = 64, lr: float = 2e-3, seed: int = 0):
         """Standardise the inputs on X, then Adam over shuffled minibatches
         with a cosine learning-rate decay.  Returns the per-epoch mean loss."""
         import torch
@@ -445,96 +511,129 @@ def random_k_regret(glups: np.ndarray, k: int) -> float:
     return e - 1.0
 
 
-def top_k_outcome(glups: np.ndarray, ms: np.ndarray, order: np.ndarray, k: int):
-    """(regret, tuning-time fraction) of measuring the first k of `order`."""
+def shape_cost_ms(ms: np.ndarray, steps: np.ndarray) -> np.ndarray:
+    """GPU time measure_sweep spends on each shape: the warm-up step plus
+    the timed steps (one step alone for a slow shape)."""
+    ms = np.asarray(ms, dtype=np.float64)
+    steps = np.asarray(steps)
+    return ms * np.where(steps > 1, 1 + steps, 1)
+
+
+def top_k_outcome(glups: np.ndarray, cost: np.ndarray, order: np.ndarray, k: int):
+    """(regret, tuning-time fraction) of measuring the first k of `order`:
+    regret = sweep best / best of the k - 1, time = their measuring cost
+    over the whole sweep's."""
     top = order[:k]
     regret = glups.max() / glups[top].max() - 1.0
-    frac = ms[top].sum() / ms.sum()
+    frac = cost[top].sum() / cost.sum()
     return regret, frac
 
 
-def _folds(keys, folds: int, seed: int):
-    rng = np.random.default_rng(seed)
-    keys = list(keys)
-    rng.shuffle(keys)
-    return [keys[i::folds] for i in range(folds)]
+def shape_scores(corpus: Corpus):
+    """Mean throughput of each block shape relative to the best measured
+    shape of its (kernel, grid) group: the model-free ranking."""
+    score, cnt = {}, {}
+    for idx in corpus.groups.values():
+        best = corpus.glups[idx].max()
+        for i in idx:
+            s = corpus.shape[i]
+            score[s] = score.get(s, 0.0) + corpus.glups[i] / best
+            cnt[s] = cnt.get(s, 0) + 1
+    return {s: score[s] / cnt[s] for s in score}
 
 
-def evaluate(corpus: Corpus, folds: int = 5, k: int = 8, seed: int = 0, epochs: int = 60, hidden: int = 128,
-             log=None):
-    """Leave-kernels-out cross-validation: the kernels are split into
-    `folds` groups; for each, a model trained on the other kernels' sweeps
-    ranks the held-out kernels' candidates (every grid size) and the top k
-    are 'measured' (looked up in the sweep).  Reports regret against the
-    exhaustive sweep's best and the fraction of the sweep's timed work the
-    top k cost, next to two model-free strategies: k shapes at random
-    (expected value) and the k shapes with the best mean relative throughput
-    over the training kernels."""
-    per_group = []
+def evaluate(train: Corpus, heldout: Corpus, k: int = 8, seed: int = 0, epochs: int = 60, hidden: int = 128,
+             model: "CostModel | None" = None, log=None):
+    """Train the cost model on `train` (or use `model`), then for every
+    held-out (kernel, grid) group rank its candidates, 'measure' the top k
+    (look them up in the group's exhaustive sweep) and record the regret
+    against the sweep's best and the fraction of the sweep's GPU time the
+    k measurements plus the model's inference cost.  Next to it: two
+    model-free strategies with the same k, k shapes at random (expected
+    regret) and the k shapes with the best mean relative throughput over
+    the training groups."""
     fit_s = 0.0
-    for fi, held in enumerate(_folds(corpus.kernels(), folds, seed)):
-        held = set(held)
-        tr = np.array([i for i, r in enumerate(corpus.rows) if r["key"] not in held])
-        m = CostModel(hidden=hidden, seed=seed + 1 + fi)
+    own = model is None
+    if own:
+        model = CostModel(hidden=hidden, seed=seed + 1)
         t0 = time.perf_counter()
-        m.fit(corpus.X[tr], corpus.y[tr], epochs=epochs, seed=seed + fi)
-        fit_s += time.perf_counter() - t0
-        # model-free baseline: mean throughput relative to each training sweep's best
-        score, cnt = {}, {}
-        for gk, idx in corpus.groups.items():
-            if gk[0] in held:
-                continue
-            best = corpus.glups[idx].max()
-            for i in idx:
-                s = corpus.shape[i]
-                score[s] = score.get(s, 0.0) + corpus.glups[i] / best
-                cnt[s] = cnt.get(s, 0) + 1
-        for gk, idx in corpus.groups.items():
-            if gk[0] not in held:
-                continue
-            idx = np.array(idx)
-            g = corpus.glups[idx]
-            ms = corpus.ms[idx]
-            t0 = time.perf_counter()
-            pred = m.predict(corpus.X[idx])
-            infer_s = time.perf_counter() - t0
-            reg, frac = top_k_outcome(g, ms, np.argsort(-pred, kind="stable"), k)
-            fixed = np.argsort(-np.array([score.get(corpus.shape[i], 0.0) / max(1, cnt.get(corpus.shape[i], 0))
-                                          for i in idx]), kind="stable")
-            freg, ffrac = top_k_outcome(g, ms, fixed, k)
-            per_group.append(dict(kernel=gk[0], size=list(gk[1:]), candidates=len(idx), best_glups=float(g.max()),
-                                  model_regret=reg, model_time_frac=frac, random_regret=random_k_regret(g, k),
-                                  fixed_regret=freg, fixed_time_frac=ffrac, inference_s=infer_s))
-        m.close()
+        model.fit(train.X, train.y, epochs=epochs, seed=seed)
+        fit_s = time.perf_counter() - t0
         if log:
-            log(f"fold {fi + 1}/{folds}: {len(held)} kernels held out")
+            log(f"trained on {len(train.rows)} rows from {len(train.groups)} groups in {fit_s:.1f}s")
+    scores = shape_scores(train)
+    per_group = []
+    spent = {"model": 0.0, "fixed": 0.0, "sweep": 0.0}
+    for gk, idx in heldout.groups.items():
+        idx = np.array(idx)
+        g = heldout.glups[idx]
+        cost = shape_cost_ms(heldout.ms[idx], heldout.steps[idx])
+        t0 = time.perf_counter()
+        pred = model.predict(heldout.X[idx])
+        order = np.argsort(-pred, kind="stable")
+        infer_ms = (time.perf_counter() - t0) * 1e3
+        reg, frac = top_k_outcome(g, cost, order, k)
+        frac += infer_ms / cost.sum()
+        fixed = np.argsort(-np.array([scores.get(heldout.shape[i], 0.0) for i in idx]), kind="stable")
+        freg, ffrac = top_k_outcome(g, cost, fixed, k)
+        spent["model"] += cost[order[:k]].sum() + infer_ms
+        spent["fixed"] += cost[fixed[:k]].sum()
+        spent["sweep"] += cost.sum()
+        per_group.append(dict(kernel=gk[0], size=list(gk[1:]), candidates=len(idx), best_glups=float(g.max()),
+                              best_shape=list(heldout.shape[idx[int(np.argmax(g))]]),
+                              model_shape=list(heldout.shape[idx[order[:k][int(np.argmax(g[order[:k]]))]]]),
+                              model_regret=reg, model_time_frac=frac, random_regret=random_k_regret(g, k),
+                              fixed_regret=freg, fixed_time_frac=ffrac, inference_ms=infer_ms,
+                              sweep_ms=float(cost.sum()), rank_corr=float(np.corrcoef(pred, np.log(g))[0, 1])
+                              if len(idx) > 2 else 1.0))
+    if own:
+        model.close()
 
     def agg(key, sel=lambda d: True):
         v = np.array([d[key] for d in per_group if sel(d)])
-        return dict(mean=float(v.mean()), median=float(np.median(v)), max=float(v.max())) if v.size else None
+        return dict(mean=float(v.mean()), median=float(np.median(v)), p90=float(np.quantile(v, 0.9)),
+                    max=float(v.max())) if v.size else None
 
-    big = lambda d: d["candidates"] > 4 * k   # noqa: E731
-    return dict(k=k, folds=folds, groups=len(per_group), kernels=len(corpus.kernels()), rows=len(corpus.rows),
-                fit_seconds_total=fit_s,
+    return dict(k=k, train_rows=len(train.rows), train_groups=len(train.groups), heldout_groups=len(per_group),
+                heldout_rows=len(heldout.rows), fit_seconds=fit_s,
                 model_regret=agg("model_regret"), random_regret=agg("random_regret"),
                 fixed_regret=agg("fixed_regret"), model_time_frac=agg("model_time_frac"),
-                model_regret_large=agg("model_regret", big), random_regret_large=agg("random_regret", big),
-                fixed_regret_large=agg("fixed_regret", big), large_groups=sum(1 for d in per_group if big(d)),
+                fixed_time_frac=agg("fixed_time_frac"), rank_corr=agg("rank_corr"),
+                # GPU time of tuning all held-out configurations, over the exhaustive sweeps' time
+                model_time_frac_total=spent["model"] / spent["sweep"] if spent["sweep"] else None,
+                fixed_time_frac_total=spent["fixed"] / spent["sweep"] if spent["sweep"] else None,
+                sweep_seconds_total=spent["sweep"] * 1e-3,
+                within_5pct=float(np.mean([d["model_regret"] <= 0.05 for d in per_group])) if per_group else None,
                 per_group=per_group)
 
 
 # ------------------------------------------------------------------- tuning
-def tune(kernel, size, k: int = 8, model: CostModel | None = None, d0=None, d1=None, steps: int = 5):
-    """Rank the kernel's candidate block shapes with the cost model, time
-    the top k on the grid and return (best block, GLup/s, timings)."""
+def rank(kernel, size, model: "CostModel"):
+    """The kernel's launchable sweep shapes, best predicted first."""
+    desc = kernel.desc
+    shapes = candidates(kernel, desc.dimension)
+    occ = {}
+    X = []
+    for s in shapes:
+        t = s[0] * s[1] * s[2]
+        if t not in occ:
+            occ[t] = kernel.resources(t)
+        X.append(features(desc, size, s, occ[t]))
+    pred = model.predict(np.stack(X))
+    return [shapes[i] for i in np.argsort(-pred, kind="stable")]
+
+
+def tune(kernel, size, k: int = 8, model: "CostModel | None" = None, d0=None, d1=None, steps: int = 2):
+    """Block shape for `kernel` on a grid of `size` (incl. halo): rank the
+    sweep's shapes with the cost model and time the top k, or, with no
+    trained model in the tree, time every shape (the reference's exhaustive
+    sweep).  Returns (best block, GLup/s, [(block, GLup/s) measured])."""
     import torch
 
     from . import runtime as rt
-    model = model or CostModel.load()
-    desc = kernel.desc
-    shapes = candidates(kernel, desc.dimension)
-    pred = model.predict(np.stack([features(desc, size, s) for s in shapes]))
-    top = [shapes[i] for i in np.argsort(-pred, kind="stable")[:k]]
+    if model is None and os.path.exists(DEFAULT_MODEL):
+        model = CostModel.load()
+    shapes = rank(kernel, size, model)[:k] if model is not None else candidates(kernel, kernel.desc.dimension)
     g = kernel.geometry(size)
     grids = None
     if d0 is None:
@@ -543,7 +642,7 @@ def tune(kernel, size, k: int = 8, model: CostModel | None = None, d0=None, d1=N
             kernel.init(g, grids.ptr(i), rt.INIT_U01, 1 + i)
         d0, d1 = grids.ptr(0), grids.ptr(1)
     res = []
-    for s in top:
+    for s in shapes:
         ms = kernel.time_steps(g, s, rt.grid_blocks(g, s), d0, d1, 1, steps) / steps
         res.append((s, g.interior_points / (ms * 1e-3) / 1e9))
     del grids
@@ -553,22 +652,36 @@ def tune(kernel, size, k: int = 8, model: CostModel | None = None, d0=None, d1=N
 
 
 # ---------------------------------------------------------------------- CLI
+def _done_groups(path):
+    if not os.path.exists(path):
+        return set()
+    try:
+        return {(r["key"], int(r["sx"]), int(r["sy"]), int(r["sz"])) for r in read_rows(path)}
+    except (EOFError, OSError):   # a gzip member cut off by a killed run
+        raise SystemExit(f"{path} is truncated; remove its last member or the file")
+
+
 def main(argv=None):
     import argparse
     ap = argparse.ArgumentParser(prog="tuner")
     sub = ap.add_subparsers(dest="cmd", required=True)
     sub.add_parser("build")
     c = sub.add_parser("collect")
-    c.add_argument("out")
-    c.add_argument("--part", default="0/1", help="i/n: measure every n-th corpus entry starting at i")
+    c.add_argument("set", choices=("train", "heldout"))
+    c.add_argument("--out", default=None)
+    c.add_argument("--budget", type=float, default=600.0, help="stop starting new groups after this many seconds")
     e = sub.add_parser("evaluate")
-    e.add_argument("data", nargs="?", default=DEFAULT_DATA)
+    e.add_argument("--train", default=TRAIN_DATA)
+    e.add_argument("--heldout", default=HELDOUT_DATA)
     e.add_argument("--json", default=None)
     e.add_argument("--k", type=int, default=8)
     e.add_argument("--epochs", type=int, default=60)
+    e.add_argument("--train-configs", type=int, default=0,
+                   help="train on the first N training configurations only (0: all)")
+    e.add_argument("--seed", type=int, default=0)
     f = sub.add_parser("fit")
-    f.add_argument("data", nargs="?", default=DEFAULT_DATA)
-    f.add_argument("model", nargs="?", default=DEFAULT_MODEL)
+    f.add_argument("--train", default=TRAIN_DATA)
+    f.add_argument("--model", default=DEFAULT_MODEL)
     f.add_argument("--epochs", type=int, default=60)
     a = ap.parse_args(argv)
 
@@ -577,18 +690,26 @@ def main(argv=None):
 
     if a.cmd == "build":
         from . import build as B
-        B.build_kernels([d for d, _ in corpus()], force=False)
+        B.build_kernels([d for d, _ in train_configs() + heldout_configs()], force=False)
         return 0
     if a.cmd == "collect":
         import torch
         torch.cuda.set_device(0)
-        i, n = (int(v) for v in a.part.split("/"))
-        entries = [(d, s) for d, sizes in corpus() for s in sizes][i::n]
-        for j, (d, s) in enumerate(entries):
+        out = a.out or DATA[a.set]
+        os.makedirs(os.path.dirname(os.path.abspath(out)), exist_ok=True)
+        done = _done_groups(out)
+        todo = [(j, d, s) for j, (d, s) in enumerate(config_set(a.set))
+                if (d.key(), *(tuple(s) + (1,) * (3 - len(s)))) not in done]
+        log(f"{a.set}: {len(done)} groups measured, {len(todo)} to go")
+        t_start = time.perf_counter()
+        for j, d, s in todo:
+            if time.perf_counter() - t_start > a.budget:
+                log("budget reached")
+                break
             t0 = time.perf_counter()
-            log(f"[{j + 1}/{len(entries)}] {d.key()} size {s} ...")
-            rows = measure_sweep(d, s)
-            write_rows(a.out, rows, append=True)
+            log(f"[{j}] {d.key()} size {s} ...")
+            rows = measure_sweep(d, s, sample=(TRAIN_SHAPES_PER_CONFIG, 1000 + j) if a.set == "train" else None)
+            write_rows(out, rows, append=True)
             best = max(rows, key=lambda r: r["glups"])
             log(f"    {len(rows)} shapes in {time.perf_counter() - t0:.1f}s, best {best['glups']:.1f} GLup/s "
                 f"at {best['bx']}x{best['by']}x{best['bz']}")
@@ -596,8 +717,11 @@ def main(argv=None):
     if a.cmd == "evaluate":
         import torch
         torch.cuda.set_device(0)
-        cp = Corpus(read_rows(a.data))
-        rep = evaluate(cp, k=a.k, epochs=a.epochs, log=log)
+        rows = read_rows(a.train)
+        if a.train_configs:
+            keep = {d.key() for d, _ in train_configs()[:a.train_configs]}
+            rows = [r for r in rows if r["key"] in keep]
+        rep = evaluate(Corpus(rows), Corpus(read_rows(a.heldout)), k=a.k, epochs=a.epochs, seed=a.seed, log=log)
         short = {kk: v for kk, v in rep.items() if kk != "per_group"}
         log(json.dumps(short, indent=1))
         if a.json:
@@ -607,7 +731,7 @@ def main(argv=None):
     if a.cmd == "fit":
         import torch
         torch.cuda.set_device(0)
-        cp = Corpus(read_rows(a.data))
+        cp = Corpus(read_rows(a.train))
         m = CostModel()
         losses = m.fit(cp.X, cp.y, epochs=a.epochs)
         m.save(a.model)

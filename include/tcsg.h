@@ -166,10 +166,12 @@ int tcsg_time_steps(tcsg_kernel *k, const tcsg_geometry *g, tcsg_dim3 block, tcs
 /* ---- batches of independent grids (kernels generated with the batched
  * option; no reference counterpart) ----
  * One persistent kernel per time step advances every grid of the batch: its
- * workgroups stride over a tile table whose entries are (grid, block) pairs,
- * each the work of one block of tcsg_update.  geoms[i] is grid i's
- * geometry, bufs0[i]/bufs1[i] its ping-pong buffers; every tile uses `block`;
- * workgroups = 0 picks 8 per compute unit (at most one per tile). */
+ * workgroups walk a tile table whose entries are (grid, block) pairs, each
+ * the work of one block of tcsg_update, the workgroups of each XCD through
+ * one contiguous eighth of the table.  geoms[i] is grid i's geometry,
+ * bufs0[i]/bufs1[i] its ping-pong buffers; every tile uses `block`;
+ * workgroups = 0 launches as many workgroups per compute unit as the
+ * kernel's occupancy allows at that block size (at most one per tile). */
 typedef struct tcsg_batch tcsg_batch;
 int tcsg_batch_create(tcsg_batch **out, tcsg_kernel *k, uint32_t n, const tcsg_geometry *geoms,
                       void *const *bufs0, void *const *bufs1, tcsg_dim3 block, uint32_t workgroups);
@@ -186,11 +188,13 @@ int tcsg_batch_run_separate(tcsg_batch *b, int steps, void *stream, int *final_i
 /* Total interior points of the batch and its tile count. */
 int tcsg_batch_info(const tcsg_batch *b, uint64_t *interior_points, uint32_t *tiles, uint32_t *workgroups);
 
-/* Error of an F32C state against an FP64 state over the interior, computed
- * in FP64 on the GPU: out[0] = max |a-b|, out[1] = max |b|,
- * out[2] = sum (a-b)^2, out[3] = sum b^2, out[4] = max |a-b|/max(|b|, floor).
- * Synchronous. */
-int tcsg_error_f32_vs_f64(const tcsg_geometry *g, const float *a, const double *b,
+/* Error of an F32C state `a` (layout ga) against an FP64 state `b` (layout
+ * gb) over their common interior, computed in FP64 on the GPU:
+ * out[0] = max |a-b|, out[1] = max |b|, out[2] = sum (a-b)^2,
+ * out[3] = sum b^2, out[4] = max |a-b|/max(|b|, floor).  The layouts may
+ * differ in padding (FP32 rows aligned to their own 128-byte lines) but must
+ * have the same interior.  Synchronous. */
+int tcsg_error_f32_vs_f64(const tcsg_geometry *ga, const float *a, const tcsg_geometry *gb, const double *b,
                           double floor, double *out);
 
 /* FP64 -> FP32 copy of a whole grid (round to nearest).  Asynchronous. */

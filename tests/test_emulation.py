@@ -61,9 +61,12 @@ def test_emulated_stream_kernel_matches_oracle(case):
 
 
 @pytest.mark.parametrize("dtype", ["f64", "f32c"])
-def test_emulated_batched_kernel_matches_oracle(dtype):
+@pytest.mark.parametrize("nwg", [5, 11])
+def test_emulated_batched_kernel_matches_oracle(dtype, nwg):
     """The persistent kernel over a tile table of uneven grids (fewer
-    workgroups than tiles, so each workgroup loops) equals the oracle per grid."""
+    workgroups than tiles, so each workgroup loops) equals the oracle per grid:
+    with fewer than 8 workgroups, which stride the whole table, and with 11,
+    which split it into eight per-XCD ranges of unequal workgroup counts."""
     from threadcoarsenedstencilgenerator_amd import runtime as rt
     d = manifest.cfg4_desc(shape="dppw", batched=True, dtype=F32C if dtype == "f32c" else spec.F64)
     sizes = rt.batch_sizes(4, 7, 8, 24, d.coarsening, d.range)
@@ -80,7 +83,7 @@ def test_emulated_batched_kernel_matches_oracle(dtype):
         a0.append(a.copy())
         a1.append(a.copy())
         refs.append(run(d, og, None, a.copy(), a.copy(), 3))
-    got, bad = emulate.run_batched(d, geoms, a0, a1, None, 3, (64, 2, 1), 5)
+    got, bad = emulate.run_batched(d, geoms, a0, a1, None, 3, (64, 2, 1), nwg)
     assert bad == 0
     for g, r in zip(got, refs):
         ui = np.uint32 if dt == np.float32 else np.uint64

@@ -13,11 +13,15 @@ pytestmark = pytest.mark.gpu
 
 @pytest.mark.parametrize("dtype", [F64, F32C])
 @pytest.mark.parametrize("mode", ["graph", "launches", "separate"])
-def test_batched_matches_oracle(dtype, mode):
+@pytest.mark.parametrize("shape,block,workgroups", [("dppw", (64, 2, 1), 7), ("plain", (32, 4, 2), 7),
+                                                    ("plain", (32, 4, 2), 0), ("plain", (32, 4, 2), -1)])
+def test_batched_matches_oracle(dtype, mode, shape, block, workgroups):
+    """workgroups: 7 (fewer than 8: every workgroup strides the whole table),
+    0 (as many per CU as fit: the per-XCD ranges), -1 (one per tile)."""
     import torch
 
     from threadcoarsenedstencilgenerator_amd import runtime as rt
-    d = manifest.cfg4_desc(shape="dppw", batched=True, dtype=dtype)
+    d = manifest.cfg4_desc(shape=shape, batched=True, dtype=dtype, max_threads=1024 if shape == "plain" else 256)
     k = rt.Kernel(d)
     sizes = rt.batch_sizes(6, 3, 8, 40, d.coarsening, d.range)
     print(f"batched {mode} {d.key()} sizes {sizes}", flush=True)
@@ -31,7 +35,9 @@ def test_batched_matches_oracle(dtype, mode):
         grids.append(gr)
     torch.cuda.synchronize()
     init = [gr.host(0) for gr in grids]
-    b = rt.Batch(k, geoms, [gr.ptr(0) for gr in grids], [gr.ptr(1) for gr in grids], (64, 2, 1), workgroups=7)
+    if workgroups < 0:
+        workgroups = sum(int(np.prod(rt.grid_blocks(g, block))) for g in geoms)
+    b = rt.Batch(k, geoms, [gr.ptr(0) for gr in grids], [gr.ptr(1) for gr in grids], block, workgroups=workgroups)
     steps = 5
     if mode == "separate":
         fin = b.run_separate(steps)

@@ -5,7 +5,7 @@ import subprocess
 
 import pytest
 
-from threadcoarsenedstencilgenerator_amd import build, manifest
+from threadcoarsenedstencilgenerator_amd import build, manifest, tuned
 
 pytestmark = pytest.mark.gpu
 
@@ -31,7 +31,7 @@ def test_bench_sweep_output_format():
 
 
 def test_bench_single_block_and_mismatch():
-    d = manifest.cfg2_desc(2, 4, 128, shape="dppw", max_threads=256).replace(prefetch=0)
+    d = tuned.CFG2_F64["desc"]
     r = _bench(d, "Star", "3D", "4", "Variable", "Row", "Write", "16", "0.0", "2", "4", "128", "136", "72", "264",
                "128", "2", "1")
     assert r.returncode == 0, r.stderr

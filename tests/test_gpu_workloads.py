@@ -1,0 +1,86 @@
+"""The benchmark workloads other than the 512^3 headline grid, at their full
+sizes, against the CPU oracle:
+
+* 2D Box Jacobi r=1, 1024x1024 interior plus a 1-point halo, coarsening 1x1,
+  Row, no alignment, interior U[0,1), halo 0, 10 time steps, compared point
+  by point (bit for bit) with the FP64 CPU loop;
+* 2D Box Variable r=8 (289 weights), 16384x8192 interior, Read alignment at
+  16 points: the FP64 and FP32-compensated kernels bit for bit against the
+  oracle over 2 steps, and the FP32 kernel's error against the FP64 kernel
+  after 40 and 50 steps.
+"""
+import numpy as np
+import pytest
+
+from threadcoarsenedstencilgenerator_amd import manifest, spec, tuned
+
+from .parity import assert_bitwise, run_case
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("shape,block", [("plain", (64, 4, 1)), ("dppw", (128, 1, 1))])
+def test_box2d_jacobi_1024_ten_steps(shape, block):
+    d = manifest.cfg1_desc(shape)
+    print(f"case {d.key()} 1026x1026 block {block} 10 steps", flush=True)
+    got, ref, _, g = run_case(d, (1026, 1026), steps=10, block=block)
+    assert g.xdim == 1026 and g.ydim == 1026
+    assert_bitwise(got, ref, d.key())
+
+
+def test_box2d_r8_16384x8192_fp64_bitexact():
+    cfg = tuned.BOX2D_R8_F64
+    d, block = cfg["desc"], tuple(cfg["block"])
+    print(f"case {d.key()} 16400x8208 block {block} 2 steps", flush=True)
+    got, ref, _, _ = run_case(d, tuned.BOX2D_R8_GRID, steps=2, block=block, mode=2, mask_seed=2)
+    assert_bitwise(got, ref, d.key())
+
+
+def test_box2d_r8_fp32c_bitexact():
+    c = tuned.BOX2D_R8_F32C
+    d, block = c["desc"], tuple(c["block"])
+    print(f"case {d.key()} 16400x8208 block {block} 2 steps", flush=True)
+    got, ref, _, _ = run_case(d, tuned.BOX2D_R8_GRID, steps=2, block=block, mode=2, mask_seed=2)
+    assert_bitwise(got, ref, d.key())
+
+
+def test_box2d_r8_fp32c_error_after_40_and_50_steps():
+    """FP32-compensated against FP64 on the GPU.  With these weights (sum w =
+    -0.004, sum |w| = 1) every step shrinks the field about sixfold: after 40
+    steps max|u| is about 1e-31 and the error is bounded as for the 512^3
+    workload (normwise max <= 1e-6); after 50 steps max|u| is about 5e-39,
+    below FP32's smallest normal number (1.2e-38), the stored state and the
+    products w*v are subnormal and carry fewer significant bits, and the
+    error (reported, bounded only loosely) is that of subnormal FP32."""
+    import torch
+
+    from threadcoarsenedstencilgenerator_amd import runtime as rt
+    c64, c32 = tuned.BOX2D_R8_F64, tuned.BOX2D_R8_F32C
+    k64, k32 = rt.Kernel(c64["desc"]), rt.Kernel(c32["desc"])
+    b64, b32 = tuple(c64["block"]), tuple(c32["block"])
+    g = k64.geometry(tuned.BOX2D_R8_GRID)
+    gf = k32.geometry(tuned.BOX2D_R8_GRID)
+    assert (gf.size.x, gf.size.y) == (g.size.x, g.size.y)
+    mask = rt.mask_seeded(2, c64["desc"].n)
+    k64.set_mask(mask)
+    k32.set_mask(mask)
+    g64 = rt.Grids(g, np.float64)
+    for i in range(2):
+        k64.init(g, g64.ptr(i), rt.INIT_UPM1, 1)
+    g32 = rt.Grids(g, np.float32)
+    for i in range(2):
+        rt.convert_f64_to_f32(g64.ptr(0), g32.ptr(i), g.n_points)
+    errs = {}
+    for steps in (40, 10):
+        print(f"case {c32['desc'].key()} vs {c64['desc'].key()} +{steps} steps", flush=True)
+        f64 = k64.run(g, b64, rt.grid_blocks(g, b64), g64.ptr(0), g64.ptr(1), steps)
+        f32 = k32.run(gf, b32, rt.grid_blocks(gf, b32), g32.ptr(0), g32.ptr(1), steps)
+        assert f64 == f32 == 0  # even step counts end in buffer 0
+        torch.cuda.synchronize()
+        mx, mb, s2, b2, _ = rt.error_f32_vs_f64(gf, g32.ptr(0), g, g64.ptr(0), 1e-3)
+        errs[steps] = (mx / mb, float(np.sqrt(s2 / b2)), mb)
+    (e40, r40, m40), (e50, r50, m50) = errs[40], errs[10]
+    print(f"f32c vs f64: 40 steps normwise max {e40:.3e} rms {r40:.3e} (max|u| {m40:.2e}); "
+          f"50 steps normwise max {e50:.3e} rms {r50:.3e} (max|u| {m50:.2e})", flush=True)
+    assert e40 <= 1e-6
+    assert m50 < 1.2e-38 and e50 <= 1e-4

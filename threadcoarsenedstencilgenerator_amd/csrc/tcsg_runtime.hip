@@ -137,21 +137,28 @@ __device__ __forceinline__ void atomic_max_nonneg(double *p, double v)
     atomicMax((unsigned long long *)p, (unsigned long long)__double_as_longlong(v));
 }
 
-// FP64 error reduction of an FP32 state against an FP64 state over the interior.
-__global__ void error_kernel(const float *__restrict__ a, const double *__restrict__ b, uint32_t xdim,
-                             uint32_t ydim, uint32_t zdim, uint32_t lo, uint32_t ro, uint32_t r,
-                             uint32_t dim, double floor_v, double *out)
+// Layout of one grid buffer for the error reduction: row pitch, rows per
+// plane and the offset of the first interior point.
+struct GridIndex {
+    uint64_t pitch, rows, first;
+    __host__ __device__ uint64_t at(uint64_t i, uint64_t j, uint64_t k) const
+    {
+        return first + i + pitch * (j + rows * k);
+    }
+};
+
+// FP64 error reduction of an FP32 state against an FP64 state over the
+// interior (nx x ny x nz points); each state in its own padded layout.
+__global__ void error_kernel(const float *__restrict__ a, GridIndex ga, const double *__restrict__ b,
+                             GridIndex gb, uint64_t nx, uint64_t ny, uint64_t nz, double floor_v, double *out)
 {
-    const uint32_t zr = dim == 3 ? r : 0;
-    const uint64_t nx = xdim - lo - ro - 2 * r, ny = ydim - 2 * r, nz = zdim - 2 * zr;
     const uint64_t n = nx * ny * nz;
     double mx = 0.0, mb = 0.0, s2 = 0.0, b2 = 0.0, mrel = 0.0;
     for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < n;
          t += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t i = t % nx, jk = t / nx, j = jk % ny, k = jk / ny;
-        const uint64_t idx = (lo + r + i) + xdim * ((r + j) + (uint64_t)ydim * (zr + k));
-        const double bv = b[idx];
-        const double d = fabs((double)a[idx] - bv);
+        const double bv = b[gb.at(i, j, k)];
+        const double d = fabs((double)a[ga.at(i, j, k)] - bv);
         mx = fmax(mx, d);
         mb = fmax(mb, fabs(bv));
         s2 += d * d;
@@ -609,7 +616,6 @@ int tcsg_batch_create(tcsg_batch **out, tcsg_kernel *k, uint32_t n, const tcsg_g
 {
     if (!out || !k || !n || !geoms || !bufs0 || !bufs1) return TCSG_ERR_ARG;
     *out = nullptr;
-    if (k->desc.variant != TCSG_STREAM) return TCSG_ERR_MISMATCH;
     int rc = tcsg_check_block(k, block);
     if (rc) return rc;
     if (k->desc.coeff == TCSG_VARIABLE && !k->mask_set) return TCSG_ERR_NO_MASK;
@@ -653,9 +659,16 @@ int tcsg_batch_create(tcsg_batch **out, tcsg_kernel *k, uint32_t n, const tcsg_g
     b->ntiles = (uint32_t)tiles.size();
     uint32_t nwg = workgroups;
     if (!nwg) {
-        int dev = 0, cus = 256;
+        // persistent grid sized to fill the device: as many workgroups per CU as
+        // the batched kernel's registers allow at this block size
+        int dev = 0, cus = 256, per_cu = 0;
         if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        nwg = 8u * (uint32_t)cus;
+        const int threads = (int)(block.x * block.y * block.z);
+        if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, b->fn, threads, 0) != hipSuccess) {
+            (void)hipGetLastError();
+            per_cu = 0;
+        }
+        nwg = (uint32_t)cus * (uint32_t)std::max(1, per_cu);
     }
     b->nwg = std::min(nwg, b->ntiles);
     hipError_t e = hipSuccess;
@@ -743,15 +756,25 @@ int tcsg_batch_info(const tcsg_batch *b, uint64_t *points, uint32_t *tiles, uint
     return TCSG_OK;
 }
 
-int tcsg_error_f32_vs_f64(const tcsg_geometry *g, const float *a, const double *b, double floor_v, double *out)
+static GridIndex grid_index(const tcsg_geometry &g)
 {
-    if (!g || !a || !b || !out) return TCSG_ERR_ARG;
+    const uint64_t r = g.range, zr = g.dimension == 3 ? g.range : 0;
+    return GridIndex{g.size.x, g.size.y, (g.left_offset + r) + (uint64_t)g.size.x * (r + (uint64_t)g.size.y * zr)};
+}
+
+int tcsg_error_f32_vs_f64(const tcsg_geometry *ga, const float *a, const tcsg_geometry *gb, const double *b,
+                          double floor_v, double *out)
+{
+    if (!ga || !gb || !a || !b || !out) return TCSG_ERR_ARG;
+    if (ga->dimension != gb->dimension || ga->range != gb->range || ga->elements.x != gb->elements.x ||
+        ga->elements.y != gb->elements.y || ga->elements.z != gb->elements.z)
+        return TCSG_ERR_ARG;  // the two states must cover the same interior
     double *d = nullptr;
     TCSG_HIP(hipMalloc(&d, 5 * sizeof(double)));
     hipError_t e = hipMemset(d, 0, 5 * sizeof(double));
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(error_kernel, dim3(1024), dim3(256), 0, nullptr, a, b, g->size.x, g->size.y, g->size.z,
-                           g->left_offset, g->right_offset, g->range, g->dimension, floor_v, d);
+        hipLaunchKernelGGL(error_kernel, dim3(1024), dim3(256), 0, nullptr, a, grid_index(*ga), b, grid_index(*gb),
+                           (uint64_t)ga->elements.x, (uint64_t)ga->elements.y, (uint64_t)ga->elements.z, floor_v, d);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpy(out, d, 5 * sizeof(double), hipMemcpyDeviceToHost);

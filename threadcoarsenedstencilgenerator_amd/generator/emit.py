@@ -338,7 +338,9 @@ class _Emitter:
         R, LO = self.R, d.left_offset
         self.thread_counts()
         self.block_coords()
-        w("  if (!(gx < thx && gy < thy && gz < thz)) return;")
+        # in the persistent batched kernel the body sits in a one-pass loop, so
+        # `break` ends this tile where the plain kernel returns
+        w(f"  if (!(gx < thx && gy < thy && gz < thz)) {'break' if self.batched_body else 'return'};")
         w(f"  const size_t x = (size_t)gx * {self.XC} + {R + LO};")
         w(f"  const size_t y = (size_t)gy * {self.YC} + {R};")
         w(f"  const size_t z = (size_t)gz * {self.ZC} + {R};" if self.three else "  const size_t z = 0;")
@@ -599,11 +601,28 @@ class _Emitter:
         w(f'extern "C" __global__ void __launch_bounds__({d.max_threads}){attr} '
           f"tcsg_update_batched(const tcsg_batch_args ba)")
         w("{")
-        w("  for (uint32_t t = blockIdx.x; t < ba.ntiles; t += gridDim.x) {")
+        w("  // XCD-aware tile ranges: the hardware hands workgroup i to XCD i % 8, so the")
+        w("  // workgroups of one XCD walk one contiguous eighth of the tile table (whose")
+        w("  // neighbouring entries are neighbouring blocks of one grid, sharing halo lines")
+        w("  // in that XCD's L2).  With fewer than 8 workgroups all of them stride the table.")
+        w("  const uint32_t nwg = gridDim.x;")
+        w("  uint32_t t0 = blockIdx.x, tend = ba.ntiles, tstep = nwg;")
+        w("  if (nwg >= 8u) {")
+        w("    const uint32_t xcd = blockIdx.x & 7u;")
+        w("    t0 = (uint32_t)(((uint64_t)ba.ntiles * xcd) >> 3) + (blockIdx.x >> 3);")
+        w("    tend = (uint32_t)(((uint64_t)ba.ntiles * (xcd + 1u)) >> 3);")
+        w("    tstep = (nwg - xcd + 7u) >> 3;  // workgroups on this XCD")
+        w("  }")
+        w("  for (uint32_t t = t0; t < tend; t += tstep) {")
         w("  const tcsg_tile tl = ba.tiles[t];")
         w("  const tcsg_kargs a = ba.grids[tl.grid];")
         self.batched_body = True
-        self.body_stream()
+        if d.variant == PLAIN:
+            w("  do {")
+            self.body_plain()
+            w("  } while (0);")
+        else:
+            self.body_stream()
         self.batched_body = False
         w("  }")
         w("}")

@@ -139,7 +139,9 @@ def default_build_set():
     descs += [cfg1_desc("plain"), cfg1_desc("stream"), cfg1_desc("dppw")]
     descs += [cfg2_desc(1, 1, 1), cfg2_desc(2, 4, 8)]            # reference-shaped kernels
     descs += [smoke_desc()]
-    descs += [cfg4_desc(F64, "plain"), cfg4_desc(F64, "dppw")]
+    descs += [cfg4_desc(F64, "plain"), cfg4_desc(F64, "dppw"), cfg4_desc(F64, "plain", max_threads=1024)]
+    descs += [cfg4_desc(F64, "plain", batched=True, max_threads=1024),
+              cfg4_desc(F32C, "plain", batched=True, max_threads=1024)]
     descs += [cfg4_desc(F64, "dppw", batched=True), cfg4_desc(F32C, "dppw", batched=True),
               cfg4_desc(F32C, "plain")]
     from . import tuned

@@ -91,7 +91,8 @@ def _lib():
         L.tcsg_batch_run.argtypes = [p, i, i, p, C.POINTER(i)]
         L.tcsg_batch_run_separate.argtypes = [p, i, p, C.POINTER(i)]
         L.tcsg_batch_info.argtypes = [p, C.POINTER(u64), C.POINTER(u32), C.POINTER(u32)]
-        L.tcsg_error_f32_vs_f64.argtypes = [C.POINTER(CGeometry), p, p, C.c_double, C.POINTER(C.c_double)]
+        L.tcsg_error_f32_vs_f64.argtypes = [C.POINTER(CGeometry), p, C.POINTER(CGeometry), p, C.c_double,
+                                            C.POINTER(C.c_double)]
         L.tcsg_convert_f64_to_f32.argtypes = [p, p, u64, p]
         L.tcsg_hbm_probe.argtypes = [i, i, p, p, u64, u32, i, p, C.POINTER(C.c_float)]
         L.tcsg_mlp_num_params.argtypes = [u32, u32]
@@ -245,10 +246,11 @@ class Kernel:
         return ms.value
 
 
-def error_f32_vs_f64(g: CGeometry, a_ptr: int, b_ptr: int, floor: float = 1e-3):
-    """FP64 error reduction on the GPU: (max|a-b|, max|b|, sum (a-b)^2, sum b^2, max pointwise rel)."""
+def error_f32_vs_f64(ga: CGeometry, a_ptr: int, gb: CGeometry, b_ptr: int, floor: float = 1e-3):
+    """FP64 error reduction on the GPU of FP32 state a (layout ga) against FP64
+    state b (layout gb): (max|a-b|, max|b|, sum (a-b)^2, sum b^2, max pointwise rel)."""
     out = (C.c_double * 5)()
-    _check(_lib().tcsg_error_f32_vs_f64(C.byref(g), a_ptr, b_ptr, floor, out), "error_f32_vs_f64")
+    _check(_lib().tcsg_error_f32_vs_f64(C.byref(ga), a_ptr, C.byref(gb), b_ptr, floor, out), "error_f32_vs_f64")
     return tuple(out)
 
 

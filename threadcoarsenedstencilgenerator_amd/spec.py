@@ -63,7 +63,7 @@ class StencilDesc:
     waves_per_eu: int = 0    # STREAM: occupancy the compiler must allow (0: its own choice)
     zfirst: bool = False     # with xcd_remap: order tiles z-fastest, so z-neighbour tiles (which
     #                          share halo planes) run together on one XCD
-    batched: bool = False    # STREAM: also emit tcsg_update_batched, a persistent kernel over a
+    batched: bool = False    # also emit tcsg_update_batched, a persistent kernel over a
     #                          tile table spanning many grids
     nt_load: bool = False    # STREAM: nontemporal hint on the input loads
 
@@ -88,8 +88,6 @@ class StencilDesc:
             raise ValueError("size of alignment has to be > 0")
         if self.xext not in ("load", "dpp", "dppw"):
             raise ValueError("xext must be 'load', 'dpp' or 'dppw'")
-        if self.batched and self.variant != STREAM:
-            raise ValueError("the batched kernel is a STREAM variant")
         if self.prefetch < 0 or self.waves_per_eu < 0 or self.waves_per_eu > 8:
             raise ValueError("prefetch must be >= 0 and waves_per_eu in [0, 8]")
         if self.max_threads % 64 or not 64 <= self.max_threads <= 1024:

@@ -2,7 +2,10 @@
 
 Each entry is the code shape, coarsening and block shape that the block-shape
 and coarsening sweeps found fastest (the measurements are summarised under
-profiles/).  bench.py runs these.
+profiles/).  bench.py runs these.  Tiles are ordered x-fastest within each
+XCD's contiguous range (zfirst=False): an XCD then owns whole rows of a
+z-chunk, so no 128-byte line at a tile's x edge is fetched by two XCDs
+(HBM reads 1.20x the interior's instead of 1.34x with z-fastest order).
 """
 from __future__ import annotations
 
@@ -10,21 +13,36 @@ from . import manifest
 from .spec import F32C, F64
 
 # 3D Star Variable r=4, 512^3 interior, FP64, Write alignment at 16 points
-CFG2_F64 = dict(desc=manifest.cfg2_desc(2, 4, 128, shape="dppw", max_threads=256).replace(prefetch=0),
+CFG2_F64 = dict(desc=manifest.cfg2_desc(2, 4, 128, shape="dppw", max_threads=256).replace(prefetch=0, zfirst=False),
                 block=(128, 2, 1))
 # the same workload in FP32 storage with compensated accumulation
 # (x coarsening 2: FP32 kernels with 4 points per lane in x returned wrong values
 # in lanes 12-15 of each 16-lane group on the test machines; the xc=2 kernels and all
 # FP64 kernels are bit-exact over the full grid)
+# FP32 rows are aligned to 32 points = 128 bytes, the line size the FP64 grid's 16 points
+# make (16 FP32 points, 64 bytes, leaves each row end's line written by two tiles: 5.6%
+# more write traffic); the first group sums the 21 terms up to the output's own plane,
+# the second the 4 terms of the planes after it, and TwoSum joins them.
 CFG2_F32C = dict(desc=manifest.cfg2_desc(2, 8, 64, shape="dppw", max_threads=256, dtype=F32C).replace(
-    prefetch=0, comp_group=13), block=(128, 1, 1))
+    prefetch=0, comp_group=21, zfirst=False, cl_size=32), block=(64, 4, 1))
+
+
+# 2D Box Variable r=8 (289 weights), 16384x8192 interior, Read alignment at 16 points
+BOX2D_R8_GRID = (16384 + 16, 8192 + 16)
+# (FP64-bound: 577 flops per update; the reference's own code shape with a 4x2 register
+# tile loads each footprint point once per thread and keeps the 289 weights in SGPRs)
+BOX2D_R8_F64 = dict(desc=manifest.cfg3_desc(4, 2, shape="plain", max_threads=1024), block=(64, 8, 1))
+BOX2D_R8_F32C = dict(desc=manifest.cfg3_desc(4, 2, shape="plain", max_threads=1024, dtype=F32C).replace(
+    comp_group=17), block=(64, 8, 1))
 
 
 def best(workload: str, dtype: int = F64):
     if workload == "star3d_r4_512":
         return CFG2_F32C if dtype == F32C else CFG2_F64
+    if workload == "box2d_r8_16384x8192":
+        return BOX2D_R8_F32C if dtype == F32C else BOX2D_R8_F64
     raise KeyError(workload)
 
 
 def build_set():
-    return [CFG2_F64["desc"], CFG2_F32C["desc"]]
+    return [CFG2_F64["desc"], CFG2_F32C["desc"], BOX2D_R8_F64["desc"], BOX2D_R8_F32C["desc"]]

@@ -11,7 +11,7 @@ compiles it to a code object instead.  Options that select the MI355X code
 shape (no reference counterpart) are given as --name=value:
 --variant=plain|stream, --xext=load|dpp|dppw, --dtype=f64|f32c,
 --max-threads=N, --prefetch=N, --waves-per-eu=N, --comp-group=N,
---no-xcd-remap, --no-nt-store, --zfirst.
+--order=point|offset (plain variant), --no-xcd-remap, --no-nt-store, --zfirst.
 Invalid arguments print the usage on stderr and exit with status 1, as the
 reference does.
 """
@@ -103,7 +103,9 @@ def desc_from_argv(dim: int, argv):
         if k in opts:
             kw[f] = _int(opts[k], k, usage)
     if kw["variant"] == spec.PLAIN:
-        kw.update(xext="load", xcd_remap=False, nt_store=False, zfirst=False)
+        kw.update(xext="load", xcd_remap=False, nt_store=False, zfirst=False, order=opts.get("order", "point"))
+    elif "order" in opts:
+        _err("--order applies to the plain variant", usage)
     try:
         return spec.StencilDesc(**kw), out
     except ValueError as e:

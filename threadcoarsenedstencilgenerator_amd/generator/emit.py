@@ -162,7 +162,7 @@ class _Emitter:
           f"alignment {['None', 'Read', 'Write'][d.alignment]}/{d.cl_size} "
           f"{'f32c' if self.f32 else 'f64'} {'stream' if d.variant == STREAM else 'plain'} "
           f"xext={d.xext} xcd_remap={int(d.xcd_remap)} nt={int(d.nt_store)}"
-          + (" nt_load=1" if d.nt_load else "")
+          + (" nt_load=1" if d.nt_load else "") + (" order=offset" if d.order == "offset" else "")
           + (" (CPU emulation build)" if cpu else ""))
         if cpu:
             w('#include "cpu_emu.h"')
@@ -349,6 +349,8 @@ class _Emitter:
         w(f"  const {T}* __restrict__ src = a.in + z * sxy + y * sx + x;")
         w(f"  {T}* __restrict__ dst = a.out + z * sxy + y * sx + x;")
         self.weights()
+        if d.order == "offset":
+            return self.body_plain_offset_order()
         pts, contrib = self.footprint()
         last = {}
         for idx, p in enumerate(pts):
@@ -371,6 +373,34 @@ class _Emitter:
                     i, j, k = o
                     w(f"  dst[(ptrdiff_t)({k}) * (ptrdiff_t)sxy + (ptrdiff_t)({j}) * (ptrdiff_t)sx + {i}]"
                       f" = {self.final_expr(o)};")
+
+    def body_plain_offset_order(self):
+        """PLAIN body, offset-major: for each stencil offset in summation
+        order, the term of every output.  Each output still sums its terms in
+        the reference's order; the weights are needed one after another (so a
+        large mask streams through the scalar registers once), and a footprint
+        point is loaded just before its first use and dies after its last."""
+        T = self.T
+        w = self.w
+        loaded, rows = set(), set()
+        outs = self.outputs()
+        for t, (dx, dy, dz) in enumerate(self.offs):
+            m = spec.mask_index(self.d, dx, dy, dz)
+            for o in outs:
+                px, py, pz = o[0] + dx, o[1] + dy, o[2] + dz
+                v = f"v_{_n(px)}_{_n(py)}_{_n(pz)}"
+                if (py, pz) not in rows:
+                    rows.add((py, pz))
+                    w(f"  const {T}* __restrict__ q_{_n(py)}_{_n(pz)} = src + (ptrdiff_t)({py}) * (ptrdiff_t)sx"
+                      f" + (ptrdiff_t)({pz}) * (ptrdiff_t)sxy;")
+                if v not in loaded:
+                    loaded.add(v)
+                    w(f"  const {T} {v} = q_{_n(py)}_{_n(pz)}[{px}];")
+                w("  " + self.acc_stmt(o, t, m, v))
+        for o in outs:
+            i, j, k = o
+            w(f"  dst[(ptrdiff_t)({k}) * (ptrdiff_t)sxy + (ptrdiff_t)({j}) * (ptrdiff_t)sx + {i}]"
+              f" = {self.final_expr(o)};")
 
     def weights(self):
         if self.d.coeff != VARIABLE:

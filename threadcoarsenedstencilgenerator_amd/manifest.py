@@ -14,6 +14,7 @@ from .spec import (ALIGN_NONE, ALIGN_READ, ALIGN_WRITE, BOX, COLUMN, F32C, F64, 
 # code shapes every parity case is built in
 SHAPES = {
     "plain": dict(variant=PLAIN),
+    "plain_offset": dict(variant=PLAIN, order="offset"),
     "stream": dict(variant=STREAM, xext="load", xcd_remap=True, nt_store=True),
     "dpp": dict(variant=STREAM, xext="dpp", xcd_remap=True, nt_store=True),
     "dppw": dict(variant=STREAM, xext="dppw", xcd_remap=True, nt_store=True, zfirst=True),
@@ -56,7 +57,7 @@ def f32c_cases():
     out = []
     for dim, nb, coeff, orient in ((3, STAR, VARIABLE, ROW), (2, BOX, VARIABLE, COLUMN), (3, BOX, JACOBI, ROW),
                                    (2, STAR, JACOBI, ROW)):
-        for shape in ("plain", "dpp", "dppw"):
+        for shape in ("plain", "plain_offset", "dpp", "dppw"):
             d = StencilDesc(dimension=dim, neighbourhood=nb, orientation=orient, coeff=coeff, range=2, xc=2,
                             yc=2, zc=2 if dim == 3 else 1, alignment=ALIGN_WRITE, cl_size=16, dtype=F32C,
                             **SHAPES[shape])

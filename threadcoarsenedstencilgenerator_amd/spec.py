@@ -36,7 +36,7 @@ class StencilDesc:
     dimension/neighbourhood/orientation/coeff/range/coarsening/alignment/
     cl_size mirror the reference generator argv.  dtype, variant and the
     tuning knobs (max_threads, xext, xcd_remap, nt_store, comp_group,
-    prefetch, waves_per_eu, zfirst, batched, nt_load) have no reference counterpart: they select the
+    prefetch, waves_per_eu, zfirst, batched, nt_load, order) have no reference counterpart: they select the
     MI355X code shape.
     """
 
@@ -66,6 +66,10 @@ class StencilDesc:
     batched: bool = False    # also emit tcsg_update_batched, a persistent kernel over a
     #                          tile table spanning many grids
     nt_load: bool = False    # STREAM: nontemporal hint on the input loads
+    order: str = "point"     # PLAIN statement order: "point" loads each footprint point once and
+    #                          adds it to every output that uses it; "offset" walks the mask in
+    #                          summation order, every output per offset (weights used in order,
+    #                          each loaded point kept until its last use)
 
     def __post_init__(self):
         if self.dimension not in (2, 3):
@@ -94,6 +98,10 @@ class StencilDesc:
             raise ValueError("max_threads must be a multiple of 64 in [64, 1024]")
         if self.comp_group < 1:
             raise ValueError("comp_group must be >= 1")
+        if self.order not in ("point", "offset"):
+            raise ValueError("order must be 'point' or 'offset'")
+        if self.order == "offset" and self.variant != PLAIN:
+            raise ValueError("the offset order is a PLAIN statement order")
 
     # ---- derived quantities ----
     @property
@@ -152,7 +160,7 @@ class StencilDesc:
 
 # knobs added after kernel names were first recorded (measured corpora refer to
 # kernels by name): left out of the name's hash while at their default
-_LATER_FIELDS = {"nt_load": False}
+_LATER_FIELDS = {"nt_load": False, "order": "point"}
 
 
 def mask_index(desc: StencilDesc, dx: int, dy: int, dz: int = 0):

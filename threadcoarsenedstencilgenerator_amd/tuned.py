@@ -29,11 +29,13 @@ CFG2_F32C = dict(desc=manifest.cfg2_desc(2, 8, 64, shape="dppw", max_threads=256
 
 # 2D Box Variable r=8 (289 weights), 16384x8192 interior, Read alignment at 16 points
 BOX2D_R8_GRID = (16384 + 16, 8192 + 16)
-# (FP64-bound: 577 flops per update; the reference's own code shape with a 4x2 register
-# tile loads each footprint point once per thread and keeps the 289 weights in SGPRs)
-BOX2D_R8_F64 = dict(desc=manifest.cfg3_desc(4, 2, shape="plain", max_threads=1024), block=(64, 8, 1))
-BOX2D_R8_F32C = dict(desc=manifest.cfg3_desc(4, 2, shape="plain", max_threads=1024, dtype=F32C).replace(
-    comp_group=17), block=(64, 8, 1))
+# (FP64-bound: 577 flops per update.  The reference's code shape with a 4x4 register tile,
+# its statements in offset order: the 289 weights stream through the scalar registers once
+# per thread, each used for all 16 outputs, and each footprint point is loaded once)
+BOX2D_R8_F64 = dict(desc=manifest.cfg3_desc(4, 4, shape="plain", max_threads=256).replace(order="offset"),
+                    block=(64, 2, 1))
+BOX2D_R8_F32C = dict(desc=manifest.cfg3_desc(4, 4, shape="plain", max_threads=256, dtype=F32C).replace(
+    order="offset", comp_group=17), block=(64, 2, 1))
 
 
 def best(workload: str, dtype: int = F64):

@@ -12,7 +12,7 @@ sizes, against the CPU oracle:
 import numpy as np
 import pytest
 
-from threadcoarsenedstencilgenerator_amd import manifest, spec, tuned
+from threadcoarsenedstencilgenerator_amd import manifest, tuned
 
 from .parity import assert_bitwise, run_case
 

@@ -41,7 +41,7 @@ import time
 
 import numpy as np
 
-from .spec import (ALIGN_NONE, ALIGN_READ, ALIGN_WRITE, BOX, COLUMN, F32C, F64, JACOBI, PLAIN, ROW, STAR,
+from .spec import (ALIGN_NONE, BOX, COLUMN, F32C, F64, PLAIN, ROW, STAR,
                    STREAM, VARIABLE, StencilDesc)
 
 HERE = os.path.dirname(os.path.abspath(__file__))

@@ -1,9 +1,15 @@
 """Shared GPU-vs-oracle runner for the parity tests."""
 from __future__ import annotations
 
+import hashlib
+import json
+import os
+
 import numpy as np
 
 import oracle
+
+_GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "oracle_parity.json")
 
 
 def run_case(desc, size, steps, block, mode=None, seed=1, mask_seed=3):
@@ -46,3 +52,10 @@ def assert_bitwise(got, ref, what=""):
         idx = np.flatnonzero(~same)
         raise AssertionError(f"{what}: {idx.size} of {got.size} points differ; first at {idx[:5]}: "
                              f"gpu {got.flat[idx[0]]!r} oracle {ref.flat[idx[0]]!r}")
+
+
+def assert_golden(got, name):
+    """The GPU's final grid has the digest committed for this pinned run."""
+    with open(_GOLDEN) as f:
+        want = json.load(f)[name]["sha256"]
+    assert hashlib.sha256(np.ascontiguousarray(got).tobytes()).hexdigest() == want, f"{name}: digest differs"

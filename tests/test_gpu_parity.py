@@ -6,7 +6,7 @@ import pytest
 
 from threadcoarsenedstencilgenerator_amd import manifest, spec
 
-from .parity import assert_bitwise, run_case
+from .parity import assert_bitwise, assert_golden, run_case
 
 pytestmark = pytest.mark.gpu
 
@@ -29,6 +29,7 @@ def test_parity_f64(case):
         print(f"case {d.key()} size {size} block {block}", flush=True)
         got, ref, _, _ = run_case(d, size, steps=3, block=block)
         assert_bitwise(got, ref, f"{d.key()} block {block}")
+        assert_golden(got, d.key())
 
 
 F32 = manifest.f32c_cases()
@@ -41,3 +42,4 @@ def test_parity_f32c(case):
     print(f"case {d.key()} size {size} block {block}", flush=True)
     got, ref, _, _ = run_case(d, size, steps=3, block=block, mode=2)
     assert_bitwise(got, ref, d.key())
+    assert_golden(got, d.key())

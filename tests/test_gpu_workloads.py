@@ -14,7 +14,7 @@ import pytest
 
 from threadcoarsenedstencilgenerator_amd import manifest, tuned
 
-from .parity import assert_bitwise, run_case
+from .parity import assert_bitwise, assert_golden, run_case
 
 pytestmark = pytest.mark.gpu
 
@@ -26,6 +26,7 @@ def test_box2d_jacobi_1024_ten_steps(shape, block):
     got, ref, _, g = run_case(d, (1026, 1026), steps=10, block=block)
     assert g.xdim == 1026 and g.ydim == 1026
     assert_bitwise(got, ref, d.key())
+    assert_golden(got, "box2d_jacobi_r1_1024_10steps")
 
 
 def test_box2d_r8_16384x8192_fp64_bitexact():

@@ -35,7 +35,7 @@ BOX2D_R8_GRID = (16384 + 16, 8192 + 16)
 BOX2D_R8_F64 = dict(desc=manifest.cfg3_desc(4, 4, shape="plain", max_threads=256).replace(order="offset"),
                     block=(64, 2, 1))
 BOX2D_R8_F32C = dict(desc=manifest.cfg3_desc(4, 4, shape="plain", max_threads=256, dtype=F32C).replace(
-    order="offset", comp_group=17), block=(64, 2, 1))
+    order="offset", comp_group=17), block=(128, 2, 1))
 
 
 def best(workload: str, dtype: int = F64):

@@ -1,8 +1,8 @@
 """CPU oracle for the thread-coarsened stencil path (test infrastructure).
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s
-``cpu_baseline`` leg import this package, and only as the checker or the
-timed CPU baseline.  The product (``threadcoarsenedstencilgenerator_amd``)
+``cpu_baseline`` leg import this package, and only as the reference the
+results are compared with or the timed CPU baseline.  The product (``threadcoarsenedstencilgenerator_amd``)
 never imports it.
 
 The arithmetic lives in ``stencil_oracle.c`` (plain C + OpenMP, compiled with

@@ -1,6 +1,6 @@
 """CPU restatement of the tuner's cost-model network (test infrastructure).
 
-Only tests/ import this module, as the checker of the HIP network in
+Only tests/ import this module, as the reference for the HIP network in
 threadcoarsenedstencilgenerator_amd/csrc/tcsg_mlp.hip.  The network has no
 counterpart in the reference (hip/benchmark.cpp:221-263 measures every block
 shape; nothing there predicts one), so the arithmetic here is pinned by the

@@ -1,7 +1,7 @@
 /*
  * CPU oracle for the thread-coarsened stencil path.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker: it is loaded by the
+ * TEST INFRASTRUCTURE ONLY.  Results are compared against it: it is loaded by the
  * tests, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg, and by
  * nothing else.  The product path (threadcoarsenedstencilgenerator_amd) never
  * links or calls it.

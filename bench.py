@@ -239,6 +239,56 @@ def bench_box2d_r8(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_remainder(args):
+    """The 512^3 production kernel generated with remainder handling, on a
+    511^3 interior (no axis a multiple of its coarsening): every point
+    updated, against the same kernel without the option, which truncates
+    that grid as the reference does (z to a multiple of its 128-plane chunk),
+    and against itself on the 512^3 grid (the cost of the store masks)."""
+    import torch
+
+    from threadcoarsenedstencilgenerator_amd import runtime as rt
+    from threadcoarsenedstencilgenerator_amd import tuned
+    best = tuned.best("star3d_r4_512")
+    block = tuple(best["block"])
+    kr, kt = rt.Kernel(best["desc"].replace(remainder=True)), rt.Kernel(best["desc"])
+    mask = rt.mask_seeded(MASK_SEED, best["desc"].n)
+    stream = torch.cuda.current_stream()
+    out = {}
+    for name, k, grid in (("remainder_511", kr, (519, 519, 519)), ("truncating_511", kt, (519, 519, 519)),
+                          ("remainder_512", kr, GRID), ("truncating_512", kt, GRID)):
+        k.set_mask(mask)
+        g = k.geometry(grid)
+        grids = rt.Grids(g, k.dtype)
+        for i in range(2):
+            k.init(g, grids.ptr(i), rt.INIT_NORMAL, INPUT_SEED)
+        blocks = rt.grid_blocks(g, block)
+        k.run(g, block, blocks, grids.ptr(0), grids.ptr(1), args.warmup, stream)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        k.run(g, block, blocks, grids.ptr(0), grids.ptr(1), args.steps, stream)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        ms = k.time_steps(g, block, blocks, grids.ptr(0), grids.ptr(1), 0, args.steps, stream) / args.steps
+        user = (grid[0] - 2 * g.range) * (grid[1] - 2 * g.range) * (grid[2] - 2 * g.range)
+        out[name] = {"glups": round(g.interior_points * args.steps / dt / 1e9, 2), "kernel_ms": round(ms, 4),
+                     "interior": list(g.elements.tup()), "coverage": round(g.interior_points / user, 4),
+                     "hbm_gbs": round(16.0 * g.interior_points / (ms * 1e-3) / 1e9, 1)}
+        del grids
+        torch.cuda.empty_cache()
+    r = out["remainder_511"]
+    line = {"metric": "GLup/s, FP64 3D Star r=4 511^3 with remainder handling", "value": r["glups"],
+            "unit": "GLup/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
+            "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "3D Star Variable r=4, 511^3 interior + 4-point halo, FP64, Write alignment "
+                                   "16 points, N(0,1) values",
+                       "coarsening": list(best["desc"].coarsening), "block": list(block)},
+            "roofline": {"bound": "hbm", "achieved": r["hbm_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(r["hbm_gbs"] / HBM_PEAK_GBS, 4), "traffic": None},
+            "runs": out}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -246,7 +296,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--workload", default="star3d_r4_512", choices=("star3d_r4_512", "batched", "box2d_r8"))
+    ap.add_argument("--workload", default="star3d_r4_512", choices=("star3d_r4_512", "batched", "box2d_r8",
+                                                                    "star3d_r4_511_remainder"))
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
@@ -264,6 +315,8 @@ def main():
         return bench_batched(args)
     if args.workload == "box2d_r8":
         return bench_box2d_r8(args)
+    if args.workload == "star3d_r4_511_remainder":
+        return bench_remainder(args)
     best = tuned.best("star3d_r4_512")
     desc, block = best["desc"], tuple(best["block"])
     k = rt.Kernel(desc)

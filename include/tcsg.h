@@ -105,6 +105,19 @@ int tcsg_kernel_load(tcsg_kernel **out, const char *code_object_path, const tcsg
 int tcsg_kernel_free(tcsg_kernel *k);
 int tcsg_kernel_desc(const tcsg_kernel *k, tcsg_desc *out);
 
+/* Geometry for this kernel: tcsg_geometry_compute, except that a kernel
+ * generated with remainder handling (a STREAM option) keeps every point of an
+ * axis whose interior is not a multiple of the coarsening; threads per axis
+ * are then rounded up; the last thread in x stores only the columns inside,
+ * in y and z it starts early enough to end at the interior's end (which needs
+ * an interior of at least one coarsening on those axes: TCSG_ERR_ARG otherwise).
+ * The reference truncates such axes (hip/benchmark.cpp:139-161), which is
+ * what every other kernel does.  The launch entry points accept the geometry
+ * this returns. */
+int tcsg_kernel_geometry(const tcsg_kernel *k, tcsg_dim3 size, tcsg_geometry *out);
+/* *out = 1 for a kernel generated with remainder handling, else 0. */
+int tcsg_kernel_remainder(const tcsg_kernel *k, int *out);
+
 /* Variable-mask upload (n doubles in the layout of the reference's d_mask).
  * Replaces copyMask (hip/launchUpdateGrid.h:18, hip/generateLaunchUpdateGrid.py:77-82).
  * F32C kernels receive the FP32 rounding of each weight. */

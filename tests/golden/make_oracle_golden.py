@@ -1,6 +1,7 @@
 """Writes tests/golden/oracle_parity.json: SHA-256 digests of the CPU
 oracle's final grids for the small-grid parity matrix (every case of
-manifest.parity_cases() and manifest.f32c_cases(), 3 steps) and for the
+manifest.parity_cases(), manifest.f32c_cases() and manifest.remainder_cases(),
+3 steps) and for the
 2D Box Jacobi r=1 1024x1024 workload (10 steps), from the seeded inputs the
 GPU parity tests use.
 
@@ -38,8 +39,11 @@ def digest(a: np.ndarray) -> str:
 def final_grid(desc, size, steps, f32_mode=False):
     """Oracle final grid from the runtime's seeded fill: U[0,1) for FP64 cases,
     U[-1,1) rounded to FP32 for FP32-storage cases (the fills tests/parity.py
-    asks the GPU for), halo and padding 0; Variable masks from MASK_SEED."""
-    og = oracle.Geometry(desc.dimension, desc.range, desc.alignment, desc.cl_size, size, desc.coarsening)
+    asks the GPU for), halo and padding 0; Variable masks from MASK_SEED.
+    Remainder-handling kernels keep every interior point: the oracle runs on
+    the untruncated grid (coarsening 1)."""
+    c = (1, 1, 1) if desc.remainder else desc.coarsening
+    og = oracle.Geometry(desc.dimension, desc.range, desc.alignment, desc.cl_size, size, c)
     a = np.zeros(og.n_points)
     oracle.init_seeded(og, 1 if f32_mode else 0, INPUT_SEED, a)
     mask = oracle.mask_seeded(MASK_SEED, desc.n) if desc.coeff == spec.VARIABLE else None
@@ -53,6 +57,7 @@ def cases():
     """(name, desc, grid size, steps, FP32 fill) of every pinned run."""
     out = [(d.key(), d, size, 3, False) for d, size in manifest.parity_cases()]
     out += [(d.key(), d, size, 3, True) for d, size in manifest.f32c_cases()]
+    out += [(d.key(), d, size, 3, d.dtype == spec.F32C) for d, size in manifest.remainder_cases()]
     out.append(("box2d_jacobi_r1_1024_10steps", manifest.cfg1_desc("plain"), (1026, 1026), 10, False))
     return out
 

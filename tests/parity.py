@@ -35,7 +35,10 @@ def run_case(desc, size, steps, block, mode=None, seed=1, mask_seed=3):
     fin = k.run(g, block, rt.grid_blocks(g, block), grids.ptr(0), grids.ptr(1), steps)
     torch.cuda.synchronize()
     got = grids.host(fin)
-    og = oracle.Geometry(desc.dimension, desc.range, desc.alignment, desc.cl_size, size, desc.coarsening)
+    # remainder-handling kernels update every interior point: the oracle's
+    # untruncated geometry (coarsening 1) is theirs
+    c = (1, 1, 1) if desc.remainder else desc.coarsening
+    og = oracle.Geometry(desc.dimension, desc.range, desc.alignment, desc.cl_size, size, c)
     assert (og.xdim, og.ydim, og.zdim, og.lo, og.ro) == (g.size.x, g.size.y, g.size.z, g.left_offset,
                                                           g.right_offset)
     if desc.dtype == spec.F32C:

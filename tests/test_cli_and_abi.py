@@ -32,6 +32,16 @@ def test_generator_argv_like_reference():
         assert r.returncode == 1 and "usage" in r.stderr
     r = _gen("generate3DStencil.py", "Box", "Row", "Jacobi", "1", "1", "1")
     assert r.returncode == 1
+    # MI355X options: remainder handling (params slot 18 bit 2) and the PLAIN statement order
+    r = _gen("generate3DStencil.py", "Star", "Row", "Variable", "4", "2", "4", "8", "Write", "16", "--xext=dppw",
+             "--remainder")
+    assert r.returncode == 0 and "remainder=1" in r.stdout and "tcsg_st2(rout, sfull" in r.stdout
+    assert re.search(r"tcsg_params\[24\] = \{(\d+, ){18}5,", r.stdout)   # nt_store | remainder << 2
+    r = _gen("generate2DStencil.py", "Box", "Row", "Variable", "8", "4", "4", "--variant=plain", "--order=offset")
+    assert r.returncode == 0 and "order=offset" in r.stdout
+    for extra in (("--variant=plain", "--remainder"), ("--order=offset",)):
+        r = _gen("generate2DStencil.py", "Box", "Row", "Jacobi", "1", "2", "1", *extra)
+        assert r.returncode == 1 and "usage" in r.stderr
 
 
 def test_launch_interface_generator():

@@ -88,3 +88,92 @@ def test_emulated_batched_kernel_matches_oracle(dtype, nwg):
     for g, r in zip(got, refs):
         ui = np.uint32 if dt == np.float32 else np.uint64
         assert np.array_equal(g.view(ui), r.view(ui))
+
+
+def _remainder_cases():
+    """Grids whose interiors are not multiples of the coarsening on at least
+    one axis (every axis for some), generated with remainder handling."""
+    sizes = {0: (30, 15, 18), 1: (22, 13, 12), 2: (45, 23), 3: (40, 21), 4: (25, 13, 13)}
+    out = []
+    for d, size, blocks in CASES:
+        i = len(out) % 5
+        out.append((d.replace(remainder=True), sizes[i], blocks))
+    # four FP32 columns per thread (one 16-byte store per output row) and three
+    # FP64 columns (a 16-byte store and a single one)
+    base = dict(variant=spec.STREAM, xcd_remap=True, nt_store=True, remainder=True)
+    out.append((StencilDesc(dimension=3, neighbourhood=STAR, orientation=ROW, coeff=VARIABLE, range=2, xc=4, yc=1,
+                            zc=2, alignment=ALIGN_WRITE, cl_size=16, dtype=F32C, comp_group=4, xext="dppw", **base),
+                (27, 9, 10), [(64, 1, 1), (128, 1, 1)]))
+    out.append((StencilDesc(dimension=2, neighbourhood=BOX, orientation=COLUMN, coeff=JACOBI, range=1, xc=3, yc=2,
+                            alignment=ALIGN_NONE, cl_size=0, xext="dpp", **base), (33, 16), [(16, 4, 1), (96, 2, 1)]))
+    return out
+
+
+REMAINDER_CASES = _remainder_cases()
+
+
+@pytest.mark.parametrize("case", REMAINDER_CASES, ids=[c[0].key() for c in REMAINDER_CASES])
+def test_emulated_remainder_kernel_matches_oracle(case):
+    """Every interior point is updated, also on the last thread of an axis
+    whose interior is not a multiple of the coarsening: the oracle without
+    truncation (coarsening 1) equals the kernel bit for bit, halo included, and
+    the untruncated geometry keeps the user's extents."""
+    d, size, blocks = case
+    geo = spec.geometry(d, size)
+    og = oracle.Geometry(d.dimension, d.range, d.alignment, d.cl_size, size, (1, 1, 1))
+    assert (og.xdim, og.ydim, og.zdim, og.lo, og.ro) == (geo.xdim, geo.ydim, geo.zdim, geo.lo, geo.ro)
+    assert (geo.ydim, geo.zdim) == (tuple(size) + (1,))[1:3]
+    assert any(e % c for e, c in zip(geo.elements, d.coarsening))
+    a = np.zeros(og.n_points)
+    oracle.init_seeded(og, 1, 4, a)
+    mask = oracle.mask_seeded(6, d.n)
+    dt = np.float32 if d.dtype == F32C else np.float64
+    a = a.astype(dt)
+    run = oracle.run_f32c if d.dtype == F32C else oracle.run_f64
+    ref = run(d, og, mask, a.copy(), a.copy(), 2)
+    for block in blocks:
+        nblk = tuple((geo.threads[i] + block[i] - 1) // block[i] for i in range(3))
+        got, bad = emulate.run(d, geo, mask.astype(dt), a.copy(), a.copy(), 2, block, nblk)
+        assert bad == 0, f"{bad} accesses whose full offset does not fit in 32 bits"
+        ui = np.uint32 if dt == np.float32 else np.uint64
+        diff = np.flatnonzero(got.view(ui) != ref.view(ui))
+        assert diff.size == 0, f"block {block}: {diff.size} points differ, first {diff[:5]}"
+
+
+def test_emulated_batched_remainder_kernel_matches_oracle():
+    """The persistent kernel with remainder handling over grids of arbitrary
+    extents (none a multiple of the 2x2x4 coarsening; one interior a single
+    column wide) equals the untruncated oracle."""
+    d = manifest.cfg4_desc(shape="dppw", batched=True).replace(remainder=True)
+    r = d.range
+    sizes = [(2 * r + 5, 2 * r + 7, 2 * r + 7), (2 * r + 11, 2 * r + 3, 2 * r + 9), (2 * r + 1, 2 * r + 5, 2 * r + 6)]
+    geoms, a0, a1, refs = [], [], [], []
+    for i, size in enumerate(sizes):
+        geo = spec.geometry(d, size)
+        og = oracle.Geometry(3, r, d.alignment, d.cl_size, size, (1, 1, 1))
+        a = np.zeros(og.n_points)
+        oracle.init_seeded(og, 0, 20 + i, a)
+        geoms.append(geo)
+        a0.append(a.copy())
+        a1.append(a.copy())
+        refs.append(oracle.run_f64(d, og, None, a.copy(), a.copy(), 3))
+    got, bad = emulate.run_batched(d, geoms, a0, a1, None, 3, (64, 2, 1), 9)
+    assert bad == 0
+    for g, ref in zip(got, refs):
+        assert np.array_equal(g.view(np.uint64), ref.view(np.uint64))
+
+
+def test_remainder_geometry_limits():
+    """Remainder handling keeps the user's extents; interiors narrower than
+    the coarsening are accepted in x (the columns past the end are dropped)
+    and refused in y and z (the last thread's shifted start would precede
+    the interior)."""
+    d = manifest.cfg4_desc(shape="dppw").replace(remainder=True)
+    r = d.range
+    g = spec.geometry(d, (2 * r + 1, 2 * r + 2, 2 * r + 4))
+    assert g.elements == (1, 2, 4) and g.threads == (1, 1, 1)
+    for size in ((2 * r + 5, 2 * r + 1, 2 * r + 9), (2 * r + 5, 2 * r + 3, 2 * r + 3)):
+        with pytest.raises(ValueError):
+            spec.geometry(d, size)
+    t = spec.geometry(d.replace(remainder=False), (2 * r + 5, 2 * r + 3, 2 * r + 9))
+    assert t.elements == (4, 2, 8)

@@ -15,12 +15,12 @@ pytestmark = pytest.mark.gpu
 GRID = (520, 520, 520)
 
 
-def _run(desc, block, steps):
+def _run(desc, block, steps, grid=GRID):
     import torch
 
     from threadcoarsenedstencilgenerator_amd import runtime as rt
     k = rt.Kernel(desc)
-    g = k.geometry(GRID)
+    g = k.geometry(grid)
     mask = rt.mask_seeded(2, desc.n)
     k.set_mask(mask)
     grids = rt.Grids(g, k.dtype)
@@ -33,7 +33,8 @@ def _run(desc, block, steps):
     got = grids.host(fin)
     del grids
     torch.cuda.empty_cache()
-    og = oracle.Geometry(3, desc.range, desc.alignment, desc.cl_size, GRID, desc.coarsening)
+    og = oracle.Geometry(3, desc.range, desc.alignment, desc.cl_size, grid,
+                         (1, 1, 1) if desc.remainder else desc.coarsening)
     run = oracle.run_f32c if desc.dtype == spec.F32C else oracle.run_f64
     ref = run(desc, og, mask, a0.copy(), a0.copy(), steps)
     return got, ref
@@ -45,6 +46,21 @@ def test_production_kernel_bitexact_512(which, steps):
     cfg = tuned.CFG2_F64 if which == "f64" else tuned.CFG2_F32C
     print(f"full-size {which} {cfg['desc'].key()} block {cfg['block']} steps {steps}", flush=True)
     got, ref = _run(cfg["desc"], tuple(cfg["block"]), steps)
+    ui = np.uint32 if got.dtype == np.float32 else np.uint64
+    bad = np.flatnonzero(got.view(ui) != ref.view(ui))
+    assert bad.size == 0, f"{bad.size} points differ from the oracle"
+
+
+@pytest.mark.parametrize("which", ["f64", "f32c"])
+def test_remainder_kernel_bitexact_511(which):
+    """The production kernels generated with remainder handling on a 511^3
+    interior (no axis a multiple of the 2x4x128 / 2x8x64 coarsening): every
+    point updated, bit for bit against the untruncated oracle over 3 steps."""
+    cfg = tuned.CFG2_F64 if which == "f64" else tuned.CFG2_F32C
+    d = cfg["desc"].replace(remainder=True)
+    grid = (519, 519, 519)
+    print(f"full-size remainder {which} {d.key()} grid {grid}", flush=True)
+    got, ref = _run(d, tuple(cfg["block"]), 3, grid)
     ui = np.uint32 if got.dtype == np.float32 else np.uint64
     bad = np.flatnonzero(got.view(ui) != ref.view(ui))
     assert bad.size == 0, f"{bad.size} points differ from the oracle"

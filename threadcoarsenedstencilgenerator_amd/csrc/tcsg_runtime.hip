@@ -182,7 +182,11 @@ __global__ void error_kernel(const float *__restrict__ a, GridIndex ga, const do
     }
 }
 
-int geometry_compute(const tcsg_desc &d, tcsg_dim3 size, tcsg_geometry &g)
+// remainder: keep every point of an axis whose interior is not a multiple of
+// the coarsening (kernels generated for it drop the last thread's columns past
+// the interior in x and start its footprint early in y and z); otherwise the
+// axis is truncated to a multiple, as the reference does.
+int geometry_compute(const tcsg_desc &d, tcsg_dim3 size, tcsg_geometry &g, bool remainder = false)
 {
     g = tcsg_geometry{};
     const uint32_t r = d.range;
@@ -191,8 +195,11 @@ int geometry_compute(const tcsg_desc &d, tcsg_dim3 size, tcsg_geometry &g)
     uint32_t t[3] = {1, 1, 1}, e[3] = {1, 1, 1};
     for (uint32_t a = 0; a < d.dimension; ++a) {
         if (sz[a] <= 2 * r) return TCSG_ERR_ARG;
-        t[a] = sz[a] - (sz[a] - 2 * r) % c[a];
+        t[a] = remainder ? sz[a] : sz[a] - (sz[a] - 2 * r) % c[a];
         e[a] = t[a] - 2 * r;
+        // remainder kernels shift the last thread in y and z back by less than
+        // one coarsening, which needs an interior of at least one
+        if (remainder && a > 0 && e[a] < c[a]) return TCSG_ERR_ARG;
     }
     uint64_t xmem = t[0], lo = 0, ro = 0;
     if (d.alignment == TCSG_ALIGN_READ) {
@@ -206,7 +213,8 @@ int geometry_compute(const tcsg_desc &d, tcsg_dim3 size, tcsg_geometry &g)
     if (xmem > 0xFFFFFFFFull) return TCSG_ERR_TOO_LARGE;
     g.size = {(uint32_t)xmem, t[1], d.dimension == 3 ? t[2] : 1u};
     g.elements = {e[0], e[1], d.dimension == 3 ? e[2] : 1u};
-    g.threads = {e[0] / c[0], e[1] / c[1], d.dimension == 3 ? e[2] / c[2] : 1u};
+    g.threads = {(e[0] + c[0] - 1) / c[0], (e[1] + c[1] - 1) / c[1],
+                 d.dimension == 3 ? (e[2] + c[2] - 1) / c[2] : 1u};
     g.left_offset = (uint32_t)lo;
     g.right_offset = (uint32_t)ro;
     g.dimension = d.dimension;
@@ -230,6 +238,7 @@ struct tcsg_kernel {
     bool mask_set = false;
     int legal_max = 0;
     int xext_mode = 0;    // STREAM x-neighbour exchange (params[16]): 0 loads, 1 DPP, 2 wave-aligned DPP
+    bool remainder = false;  // STREAM kernel that masks the last thread's outputs (params[18] bit 2)
 };
 
 extern "C" {
@@ -312,6 +321,8 @@ int tcsg_kernel_load(tcsg_kernel **out, const char *path, const tcsg_desc *expec
     if (k->legal_max < 64) return fail(TCSG_ERR_BLOCK_SHAPE);
     k->xext_mode = d.variant == TCSG_STREAM ? (int)q[16] : 0;
     if (k->xext_mode > 2) return fail(TCSG_ERR_MISMATCH);
+    k->remainder = (q[18] & 4u) != 0;
+    if (k->remainder && d.variant != TCSG_STREAM) return fail(TCSG_ERR_MISMATCH);
     *out = k;
     return TCSG_OK;
 }
@@ -399,13 +410,13 @@ int tcsg_check_block(const tcsg_kernel *k, tcsg_dim3 b)
     return TCSG_OK;
 }
 
-static int check_geom(const tcsg_desc &d, const tcsg_geometry *g)
+static int check_geom(const tcsg_desc &d, const tcsg_geometry *g, bool remainder = false)
 {
     if (g->dimension != d.dimension || g->range != d.range) return TCSG_ERR_ARG;
     tcsg_geometry ref;
     // the geometry must be one this kernel's coarsening and alignment produce
     tcsg_dim3 user{g->size.x - g->left_offset - g->right_offset, g->size.y, g->size.z};
-    if (geometry_compute(d, user, ref) != TCSG_OK) return TCSG_ERR_ARG;
+    if (geometry_compute(d, user, ref, remainder) != TCSG_OK) return TCSG_ERR_ARG;
     auto eq3 = [](tcsg_dim3 a, tcsg_dim3 b) { return a.x == b.x && a.y == b.y && a.z == b.z; };
     if (!eq3(ref.size, g->size) || !eq3(ref.elements, g->elements) || !eq3(ref.threads, g->threads) ||
         ref.left_offset != g->left_offset || ref.right_offset != g->right_offset || ref.n_points != g->n_points)
@@ -426,6 +437,19 @@ int tcsg_geometry_check(const tcsg_desc *desc, const tcsg_geometry *g)
 {
     if (!desc || !g || !desc_valid(*desc)) return TCSG_ERR_ARG;
     return check_geom(*desc, g);
+}
+
+int tcsg_kernel_geometry(const tcsg_kernel *k, tcsg_dim3 size, tcsg_geometry *out)
+{
+    if (!k || !out) return TCSG_ERR_ARG;
+    return geometry_compute(k->desc, size, *out, k->remainder);
+}
+
+int tcsg_kernel_remainder(const tcsg_kernel *k, int *out)
+{
+    if (!k || !out) return TCSG_ERR_ARG;
+    *out = k->remainder ? 1 : 0;
+    return TCSG_OK;
 }
 
 int tcsg_launch_init(tcsg_kernel *k, const tcsg_geometry *g, int mode, uint64_t seed, void *a, void *stream)
@@ -472,7 +496,7 @@ static int launch_one(tcsg_kernel *k, const tcsg_geometry *g, tcsg_dim3 block, t
 static int precheck(tcsg_kernel *k, const tcsg_geometry *g, tcsg_dim3 block, tcsg_dim3 blocks)
 {
     if (!k || !g) return TCSG_ERR_ARG;
-    int rc = check_geom(k->desc, g);
+    int rc = check_geom(k->desc, g, k->remainder);
     if (rc) return rc;
     rc = tcsg_check_block(k, block);
     if (rc) return rc;
@@ -631,7 +655,7 @@ int tcsg_batch_create(tcsg_batch **out, tcsg_kernel *k, uint32_t n, const tcsg_g
     std::vector<Tile> tiles;
     const uint64_t es = elem_size(k->desc);
     for (uint32_t i = 0; i < n; ++i) {
-        rc = check_geom(k->desc, &geoms[i]);
+        rc = check_geom(k->desc, &geoms[i], k->remainder);
         if (!rc && (!bufs0[i] || !bufs1[i] || bufs0[i] == bufs1[i])) rc = TCSG_ERR_ARG;
         if (rc) { delete b; return rc; }
         const tcsg_dim3 nb = tcsg_grid_blocks(&geoms[i], block);

@@ -11,7 +11,9 @@ compiles it to a code object instead.  Options that select the MI355X code
 shape (no reference counterpart) are given as --name=value:
 --variant=plain|stream, --xext=load|dpp|dppw, --dtype=f64|f32c,
 --max-threads=N, --prefetch=N, --waves-per-eu=N, --comp-group=N,
---order=point|offset (plain variant), --no-xcd-remap, --no-nt-store, --zfirst.
+--order=point|offset (plain variant), --no-xcd-remap, --no-nt-store, --zfirst,
+--remainder (stream variant: grids whose interior is not a multiple of the
+coarsening are updated whole instead of truncated).
 Invalid arguments print the usage on stderr and exit with status 1, as the
 reference does.
 """
@@ -97,13 +99,15 @@ def desc_from_argv(dim: int, argv):
               variant=spec.VARIANT_NAMES[opts.get("variant", "stream")],
               dtype=spec.DTYPE_NAMES[opts.get("dtype", "f64")],
               xext=opts.get("xext", "dpp"), xcd_remap="no-xcd-remap" not in opts, nt_store="no-nt-store" not in opts,
-              zfirst="zfirst" in opts)
+              zfirst="zfirst" in opts, remainder="remainder" in opts)
     for k, f in (("max-threads", "max_threads"), ("prefetch", "prefetch"), ("waves-per-eu", "waves_per_eu"),
                  ("comp-group", "comp_group")):
         if k in opts:
             kw[f] = _int(opts[k], k, usage)
     if kw["variant"] == spec.PLAIN:
         kw.update(xext="load", xcd_remap=False, nt_store=False, zfirst=False, order=opts.get("order", "point"))
+        if "remainder" in opts:
+            _err("--remainder applies to the stream variant", usage)
     elif "order" in opts:
         _err("--order applies to the plain variant", usage)
     try:

@@ -163,6 +163,7 @@ class _Emitter:
           f"{'f32c' if self.f32 else 'f64'} {'stream' if d.variant == STREAM else 'plain'} "
           f"xext={d.xext} xcd_remap={int(d.xcd_remap)} nt={int(d.nt_store)}"
           + (" nt_load=1" if d.nt_load else "") + (" order=offset" if d.order == "offset" else "")
+          + (" remainder=1" if d.remainder else "")
           + (" (CPU emulation build)" if cpu else ""))
         if cpu:
             w('#include "cpu_emu.h"')
@@ -230,6 +231,15 @@ class _Emitter:
                 w("inline tcsg_emu_rsrc tcsg_rsrc(const void* p, uint32_t n) { return tcsg_make_rsrc(p, n); }")
                 w(f"inline TV tcsg_ld(const tcsg_emu_rsrc& r, const u32& off, const u32& soff) {{ return tcsg_emu_ld<{T}>(r, off, soff); }}")
                 w("inline void tcsg_st(const tcsg_emu_rsrc& r, const u32& off, const u32& soff, const TV& v) { tcsg_emu_st(r, off, soff, v); }")
+                if d.remainder and d.xc > 1:
+                    # a wide store is range-checked per dword on the hardware, as its elements are here
+                    es_ = 4 if self.f32 else 8
+                    for n in ((2, 4) if self.f32 else (2,)):
+                        if n > d.xc:
+                            continue
+                        args = ", ".join(f"const TV& v{c}" for c in range(n))
+                        body = " ".join(f"tcsg_emu_st(r, off + {c * es_}u, soff, v{c});" for c in range(n))
+                        w(f"inline void tcsg_st{n}(const tcsg_emu_rsrc& r, const u32& off, const u32& soff, {args}) {{ {body} }}")
             else:
                 bits = 32 if self.f32 else 64
                 ity = "int" if self.f32 else "__attribute__((ext_vector_type(2))) int"
@@ -244,6 +254,18 @@ class _Emitter:
                 w(f"__device__ __forceinline__ void tcsg_st(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff, {T} v) {{")
                 w(f"  __builtin_amdgcn_raw_buffer_store_b{bits}(__builtin_bit_cast({ity}, v), r, off, soff, {aux});")
                 w("}")
+                if d.remainder and d.xc > 1:
+                    # one store instruction for consecutive columns of an output row
+                    for n in ((2, 4) if self.f32 else (2,)):
+                        if n * bits > 128 or n > d.xc:
+                            continue
+                        args = ", ".join(f"{T} v{c}" for c in range(n))
+                        vals = ", ".join(f"v{c}" for c in range(n))
+                        w(f"__device__ __forceinline__ void tcsg_st{n}(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff, {args}) {{")
+                        w(f"  const __attribute__((ext_vector_type({n}))) {T} v = {{{vals}}};")
+                        w(f"  __builtin_amdgcn_raw_buffer_store_b{n * bits}("
+                          f"__builtin_bit_cast(__attribute__((ext_vector_type({n * bits // 32}))) int, v), r, off, soff, {aux});")
+                        w("}")
         w("")
 
     def emulation_runner(self):
@@ -322,11 +344,15 @@ class _Emitter:
         w = self.w
         w("  const u32 xdim = a.xdim, ydim = a.ydim;")
         w(f"  const u32 ex = xdim - a.right_offset;  // end of the x extent in use")
-        w(f"  const u32 thx = tcsg_sel(ex >= {2 * R + LO}u, (ex - {2 * R + LO}u) / {self.XC}u, 0u);")
-        w(f"  const u32 thy = tcsg_sel(ydim >= {2 * R}u, (ydim - {2 * R}u) / {self.YC}u, 0u);")
+        # with remainder handling the last thread of an axis may own fewer than
+        # its coarsening's points (rounded-up counts); otherwise the runtime
+        # only passes extents whose interiors are multiples of the coarsening
+        up = lambda c: f" + {c - 1}u" if d.remainder and c > 1 else ""
+        w(f"  const u32 thx = tcsg_sel(ex >= {2 * R + LO}u, (ex - {2 * R + LO}u{up(self.XC)}) / {self.XC}u, 0u);")
+        w(f"  const u32 thy = tcsg_sel(ydim >= {2 * R}u, (ydim - {2 * R}u{up(self.YC)}) / {self.YC}u, 0u);")
         if self.three:
             w("  const u32 zdim = a.zdim;")
-            w(f"  const u32 thz = tcsg_sel(zdim >= {2 * R}u, (zdim - {2 * R}u) / {self.ZC}u, 0u);")
+            w(f"  const u32 thz = tcsg_sel(zdim >= {2 * R}u, (zdim - {2 * R}u{up(self.ZC)}) / {self.ZC}u, 0u);")
         else:
             w("  const u32 thz = 1u;")
 
@@ -435,9 +461,22 @@ class _Emitter:
         w("  const lbool inrow = gy < thy && gz < thz;")
         w("  const lbool valid = inrow && gx < thx;")
         w(f"  const u32 x0 = gx * {XC}u + {R + LO}u;")
-        w(f"  const u32 y0 = gy * {self.YC}u;            // first row of the footprint")
-        w(f"  const u32 z0 = gz * {self.ZC}u;            // first plane of the footprint" if self.three
-          else "  const u32 z0 = 0u;")
+        if d.remainder and (self.YC > 1 or (self.three and self.ZC > 1)):
+            w("  // remainder: the last thread of an axis whose interior is not a multiple of the")
+            w("  // coarsening starts early enough to end at the interior's end; the outputs it")
+            w("  // shares with its neighbour get the same value from both (same inputs, same")
+            w("  // summation order).  The runtime requires interiors of at least one coarsening")
+            w("  // in y and z.")
+        if d.remainder and self.YC > 1:
+            w(f"  const u32 y0 = tcsg_min(gy * {self.YC}u, ydim - {2 * R + self.YC}u);  // first row of the footprint")
+        else:
+            w(f"  const u32 y0 = gy * {self.YC}u;            // first row of the footprint")
+        if not self.three:
+            w("  const u32 z0 = 0u;")
+        elif d.remainder and self.ZC > 1:
+            w(f"  const u32 z0 = tcsg_min(gz * {self.ZC}u, zdim - {2 * R + self.ZC}u);  // first plane of the footprint")
+        else:
+            w(f"  const u32 z0 = gz * {self.ZC}u;            // first plane of the footprint")
         w(f"  const u32 sx = xdim * {es}u;            // row pitch in bytes")
         w(f"  const u32 sxy = xdim * ydim * {es}u;     // plane pitch in bytes")
         w(f"  const u32 real = (z0 * ydim + y0) * xdim * {es}u + (x0 - {XL}u) * {es}u;")
@@ -451,6 +490,19 @@ class _Emitter:
         else:
             w(f"  const u32 base = tcsg_sel(valid, real, {OOB_BASE:#x}u);")
         w(f"  const u32 sbase = tcsg_sel(valid, real, {OOB_BASE:#x}u);")
+        # x positions stay lane-contiguous for the neighbour exchange, so with
+        # remainder handling the last thread in x keeps its place and drops the
+        # columns past the interior's end instead: every thread stores an output
+        # row's XC columns with wide stores at sfull (out of range on a thread
+        # with fewer columns left), and such a thread stores the columns it has
+        # one by one at spart<i> (out of range on every other thread)
+        xgroups = d.remainder and XC > 1
+        if xgroups:
+            w(f"  const u32 nvx = ex - {R}u - x0;  // interior columns from x0 on")
+            w(f"  const u32 sfull = tcsg_sel(nvx >= {XC}u, sbase + {XL * es}u, {OOB_BASE:#x}u);")
+            for i in range(XC - 1):
+                w(f"  const u32 spart{i} = tcsg_sel(nvx > {i}u && nvx < {XC}u, sbase + {(i + XL) * es}u, "
+                  f"{OOB_BASE:#x}u);")
         w("  const auto rin = tcsg_rsrc(a.in, a.in_bytes);")
         w("  const auto rout = tcsg_rsrc(a.out, a.out_bytes);")
         self.weights()
@@ -595,9 +647,27 @@ class _Emitter:
                 for (o, t, m) in sorted(contrib[p], key=lambda c: c[0]):
                     w("  " + self.acc_stmt(o, t, m, v))
                 for o in self.outputs():
-                    if last[o] == idx:
-                        i, j, kk = o
+                    if last[o] != idx:
+                        continue
+                    i, j, kk = o
+                    if not xgroups:
                         w(f"  tcsg_st(rout, sbase + {(i + XL) * es}u, {soff(j, kk)}, {self.final_expr(o)});")
+                        continue
+                    w(f"  const TV f_{i}_{j}_{kk} = {self.final_expr(o)};")
+                    if idx == max(last[(c, j, kk)] for c in range(XC)):
+                        row = [f"f_{c}_{j}_{kk}" for c in range(XC)]
+                        # wide stores of 2 or 4 elements, at most 16 bytes each
+                        c0 = 0
+                        while c0 < XC:
+                            n = max(m for m in (1, 2, 4) if m * es <= 16 and c0 + m <= XC)
+                            part = row[c0:c0 + n]
+                            if n == 1:
+                                w(f"  tcsg_st(rout, sfull + {c0 * es}u, {soff(j, kk)}, {part[0]});")
+                            else:
+                                w(f"  tcsg_st{n}(rout, sfull + {c0 * es}u, {soff(j, kk)}, {', '.join(part)});")
+                            c0 += n
+                        for c in range(XC - 1):
+                            w(f"  tcsg_st(rout, spart{c}, {soff(j, kk)}, {row[c]});")
 
     # ------------------------------------------------------------------ all
     def source(self) -> str:

@@ -109,7 +109,9 @@ void StencilBenchmark::init_benchmark()
     check(tcsg_max_block_size(kernel_, &occ, &legal), "block size query");
     max_threads_ = c_.reference_sweep ? occ : legal;
     tcsg_dim3 size{c_.size[0], c_.size[1], c_.dimension == 3 ? c_.size[2] : 1u};
-    check(tcsg_geometry_compute(&desc_, size, &geom_), "geometry");
+    // the kernel's own geometry: truncated as the reference's, or whole for a
+    // kernel generated with remainder handling
+    check(tcsg_kernel_geometry(kernel_, size, &geom_), "geometry");
     const size_t es = desc_.dtype == TCSG_F32C ? 4 : 8;
     size_t free_b = 0, total_b = 0;
     (void)hipMemGetInfo(&free_b, &total_b);

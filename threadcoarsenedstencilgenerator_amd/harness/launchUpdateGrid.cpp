@@ -31,7 +31,7 @@ bool geometry_of(const tcsg_desc &d, dim3 gs, unsigned ro, tcsg_geometry &g)
     const unsigned lo = (d.alignment == TCSG_ALIGN_WRITE && d.range % d.cl_size) ? d.cl_size - d.range % d.cl_size : 0;
     if (gs.x < lo + ro) return false;
     tcsg_dim3 user{gs.x - lo - ro, gs.y, gs.z};
-    if (tcsg_geometry_compute(&d, user, &g) != TCSG_OK) return false;
+    if (tcsg_kernel_geometry(kernel(), user, &g) != TCSG_OK) return false;
     return g.size.x == gs.x && g.size.y == gs.y && (d.dimension == 2 || g.size.z == gs.z) && g.right_offset == ro;
 }
 

@@ -65,6 +65,29 @@ def f32c_cases():
     return out
 
 
+def remainder_cases():
+    """STREAM kernels with remainder handling on grids whose interiors are
+    not multiples of the coarsening (every parity size is ragged for these
+    coarsenings), in both DPP code shapes, plus FP32C cases."""
+    out = []
+    for d, size in parity_cases():
+        if d.variant != STREAM or d.xext == "load" or d.coarsening == (1, 1, 1):
+            continue
+        if d.range == 3 and d.xc == 1:
+            continue
+        out.append((d.replace(remainder=True), size))
+    for d, size in f32c_cases():
+        if d.variant == STREAM:
+            out.append((d.replace(remainder=True), (size[0] + 1, size[1] + 2) + tuple(s + 1 for s in size[2:])))
+    # four FP32 columns per thread (16-byte row stores) and three FP64 columns
+    out.append((StencilDesc(dimension=3, neighbourhood=STAR, orientation=ROW, coeff=VARIABLE, range=2, xc=4, yc=1,
+                            zc=2, alignment=ALIGN_WRITE, cl_size=16, dtype=F32C, comp_group=4, remainder=True,
+                            **SHAPES["dppw"]), (27, 9, 10)))
+    out.append((StencilDesc(dimension=2, neighbourhood=BOX, orientation=COLUMN, coeff=JACOBI, range=1, xc=3, yc=2,
+                            alignment=ALIGN_NONE, cl_size=0, remainder=True, **SHAPES["dpp"]), (33, 16)))
+    return out
+
+
 # ---- benchmark workloads ----
 def cfg1_desc(shape="plain"):
     """2D Box Jacobi r=1, coarsening 1x1, Row, alignment None."""
@@ -136,7 +159,7 @@ def default_build_set():
     """Kernels the tests, smoke() and bench.py load.  Tuning candidate sets
     (cfg2_sweep, cfg2_tuned, cfg2_f32c_candidates, ...) are compiled on
     demand by the sweeps and cached in the same directory."""
-    descs = [d for d, _ in parity_cases()] + [d for d, _ in f32c_cases()]
+    descs = [d for d, _ in parity_cases()] + [d for d, _ in f32c_cases()] + [d for d, _ in remainder_cases()]
     descs += [cfg1_desc("plain"), cfg1_desc("stream"), cfg1_desc("dppw")]
     descs += [cfg2_desc(1, 1, 1), cfg2_desc(2, 4, 8)]            # reference-shaped kernels
     descs += [smoke_desc()]

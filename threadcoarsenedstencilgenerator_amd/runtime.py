@@ -72,6 +72,8 @@ def _lib():
         L.tcsg_kernel_load.argtypes = [C.POINTER(p), C.c_char_p, C.POINTER(CDesc)]
         L.tcsg_kernel_free.argtypes = [p]
         L.tcsg_kernel_desc.argtypes = [p, C.POINTER(CDesc)]
+        L.tcsg_kernel_geometry.argtypes = [p, Dim3, C.POINTER(CGeometry)]
+        L.tcsg_kernel_remainder.argtypes = [p, C.POINTER(i)]
         L.tcsg_set_mask.argtypes = [p, p, u32]
         L.tcsg_max_block_size.argtypes = [p, C.POINTER(i), C.POINTER(i)]
         L.tcsg_check_block.argtypes = [p, Dim3]
@@ -213,7 +215,17 @@ class Kernel:
         return v.value, s.value, w.value
 
     def geometry(self, size) -> CGeometry:
-        return geometry(self.desc, size)
+        """Geometry this kernel launches on: the reference's truncated one, or
+        with remainder handling every point of `size`."""
+        g = CGeometry()
+        _check(_lib().tcsg_kernel_geometry(self._h, dim3(size), C.byref(g)), "kernel_geometry")
+        return g
+
+    @property
+    def remainder(self) -> bool:
+        v = C.c_int()
+        _check(_lib().tcsg_kernel_remainder(self._h, C.byref(v)), "kernel_remainder")
+        return bool(v.value)
 
     def init(self, g: CGeometry, ptr: int, mode=INIT_REFERENCE, seed=1, stream=None):
         """launchInit (hip/generateLaunchUpdateGrid.py:150-158) and seeded fills."""

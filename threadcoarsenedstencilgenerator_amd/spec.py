@@ -36,7 +36,7 @@ class StencilDesc:
     dimension/neighbourhood/orientation/coeff/range/coarsening/alignment/
     cl_size mirror the reference generator argv.  dtype, variant and the
     tuning knobs (max_threads, xext, xcd_remap, nt_store, comp_group,
-    prefetch, waves_per_eu, zfirst, batched, nt_load, order) have no reference counterpart: they select the
+    prefetch, waves_per_eu, zfirst, batched, nt_load, order, remainder) have no reference counterpart: they select the
     MI355X code shape.
     """
 
@@ -66,6 +66,10 @@ class StencilDesc:
     batched: bool = False    # also emit tcsg_update_batched, a persistent kernel over a
     #                          tile table spanning many grids
     nt_load: bool = False    # STREAM: nontemporal hint on the input loads
+    remainder: bool = False  # STREAM: grids whose interior is not a multiple of the coarsening
+    #                          keep every point (the last thread in x drops the columns past
+    #                          the interior, in y and z it starts early enough to end at the
+    #                          interior's end); off, axes are truncated as the reference does
     order: str = "point"     # PLAIN statement order: "point" loads each footprint point once and
     #                          adds it to every output that uses it; "offset" walks the mask in
     #                          summation order, every output per offset (weights used in order,
@@ -98,6 +102,8 @@ class StencilDesc:
             raise ValueError("max_threads must be a multiple of 64 in [64, 1024]")
         if self.comp_group < 1:
             raise ValueError("comp_group must be >= 1")
+        if self.remainder and self.variant != STREAM:
+            raise ValueError("remainder handling is a STREAM option")
         if self.order not in ("point", "offset"):
             raise ValueError("order must be 'point' or 'offset'")
         if self.order == "offset" and self.variant != PLAIN:
@@ -151,7 +157,7 @@ class StencilDesc:
                 self.coeff, self.range, self.xc, self.yc, self.zc, self.alignment,
                 self.cl_size, self.dtype, self.variant, self.n, self.max_threads,
                 {"load": 0, "dpp": 1, "dppw": 2}[self.xext], int(self.xcd_remap),
-                int(self.nt_store) | int(self.nt_load) << 1,
+                int(self.nt_store) | int(self.nt_load) << 1 | int(self.remainder) << 2,
                 self.comp_group, self.prefetch, self.waves_per_eu, int(self.zfirst), int(self.batched)]
 
     def replace(self, **kw) -> "StencilDesc":
@@ -160,7 +166,7 @@ class StencilDesc:
 
 # knobs added after kernel names were first recorded (measured corpora refer to
 # kernels by name): left out of the name's hash while at their default
-_LATER_FIELDS = {"nt_load": False, "order": "point"}
+_LATER_FIELDS = {"nt_load": False, "order": "point", "remainder": False}
 
 
 def mask_index(desc: StencilDesc, dx: int, dy: int, dz: int = 0):
@@ -237,7 +243,8 @@ class Geometry:
     xdim..    padded allocation extents (x includes left/right padding)
     lo, ro    left / right padding in points
     elements  interior (updated) points per axis
-    threads   threads per axis = elements / coarsening
+    threads   threads per axis = elements / coarsening, rounded up (only
+              remainder-handling kernels see interiors that are not multiples)
     """
 
     dimension: int
@@ -269,8 +276,10 @@ def geometry(desc: StencilDesc, size) -> Geometry:
     for a in range(desc.dimension):
         if size[a] <= 2 * r:
             raise ValueError(f"grid axis {a} ({size[a]}) too small for range {r}")
-        t[a] = size[a] - (size[a] - 2 * r) % c[a]
+        t[a] = size[a] if desc.remainder else size[a] - (size[a] - 2 * r) % c[a]
         e[a] = t[a] - 2 * r
+        if desc.remainder and a > 0 and e[a] < c[a]:
+            raise ValueError("remainder handling needs an interior of at least one coarsening in y and z")
     lo = ro = 0
     xmem = t[0]
     cl = desc.cl_size
@@ -284,6 +293,6 @@ def geometry(desc: StencilDesc, size) -> Geometry:
         if xmem % cl:
             ro = cl - xmem % cl
         xmem += ro
-    thr = tuple(e[a] // c[a] if a < desc.dimension else 1 for a in range(3))
+    thr = tuple(-(-e[a] // c[a]) if a < desc.dimension else 1 for a in range(3))
     return Geometry(desc.dimension, r, xmem, t[1], t[2] if desc.dimension == 3 else 1,
                     lo, ro, tuple(e), thr)

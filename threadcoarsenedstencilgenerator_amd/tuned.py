@@ -47,4 +47,5 @@ def best(workload: str, dtype: int = F64):
 
 
 def build_set():
-    return [CFG2_F64["desc"], CFG2_F32C["desc"], BOX2D_R8_F64["desc"], BOX2D_R8_F32C["desc"]]
+    return [CFG2_F64["desc"], CFG2_F32C["desc"], BOX2D_R8_F64["desc"], BOX2D_R8_F32C["desc"],
+            CFG2_F64["desc"].replace(remainder=True), CFG2_F32C["desc"].replace(remainder=True)]

@@ -120,7 +120,9 @@ def bench_batched(args):
     XCD's workgroups walking their eighth of the tile table) over a captured
     hipGraph of the step loop (the headline of this workload), the same
     without the graph, one workgroup per tile under the graph, and the
-    baseline: 64 separate launches of the plain kernel per step."""
+    baseline: 64 separate launches of the plain kernel per step.  The
+    *_unrounded entries run the same draws without the rounding (the kernel
+    generated with remainder handling)."""
     import numpy as np
     import torch
 
@@ -130,10 +132,13 @@ def bench_batched(args):
     out = {}
     steps = 100
     block = (32, 4, 2)
-    for name, dt in (("f64", F64), ("f32c", F32C)):
+    for name, dt, rem in (("f64", F64, False), ("f32c", F32C, False), ("f64_unrounded", F64, True),
+                          ("f32c_unrounded", F32C, True)):
         d = manifest.cfg4_desc(shape="plain", batched=True, dtype=dt, max_threads=1024)
+        if rem:
+            d = d.replace(remainder=True)
         k = rt.Kernel(d)
-        sizes = rt.batch_sizes(64, 1, 32, 256, d.coarsening, d.range)
+        sizes = rt.batch_sizes(64, 1, 32, 256, (1, 1, 1) if rem else d.coarsening, d.range)
         geoms, grids = [], []
         for i, size in enumerate(sizes):
             g = k.geometry(size)

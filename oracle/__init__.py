@@ -38,10 +38,12 @@ def _host_has_v3() -> bool:
 
 def build(ref: bool = True) -> None:
     """Compile the oracle (and, when the reference tree is present, the
-    reference's own primes.cpp into oracle/_ref)."""
-    subprocess.run(["make", "-s", "-C", HERE, "all"], check=True)
-    if ref and os.path.isdir("/root/reference/hip"):
-        subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
+    reference's own primes.cpp, and its harness over the compiled-in launch
+    units the package build wrote, into oracle/_ref)."""
+    for target in ("all", "ref") if ref and os.path.isdir("/root/reference/hip") else ("all",):
+        r = subprocess.run(["make", "-s", "-C", HERE, target], capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"make {target} in {HERE} failed:\n{r.stdout}\n{r.stderr}")
 
 
 def lib():

@@ -39,13 +39,20 @@ def test_generator_argv_like_reference():
     assert re.search(r"tcsg_params\[24\] = \{(\d+, ){18}5,", r.stdout)   # nt_store | remainder << 2
     r = _gen("generate2DStencil.py", "Box", "Row", "Variable", "8", "4", "4", "--variant=plain", "--order=offset")
     assert r.returncode == 0 and "order=offset" in r.stdout
-    for extra in (("--variant=plain", "--remainder"), ("--order=offset",)):
+    for extra in (("--order=offset",),):
         r = _gen("generate2DStencil.py", "Box", "Row", "Jacobi", "1", "2", "1", *extra)
         assert r.returncode == 1 and "usage" in r.stderr
 
 
 def test_launch_interface_generator():
+    # default: the reference's launchUpdateGrid.cu layout, kernels.cu compiled in, parameters fixed
     r = _gen("generateLaunchUpdateGrid.py", "Star", "Variable", "4", "3D", "Write")
+    assert r.returncode == 0 and '#include "kernels.cu"' in r.stdout and '#include "launchUpdateGrid.h"' in r.stdout
+    assert 'alignment == "Write"' in r.stdout and "stencilRange == 4u" in r.stdout and "@" not in r.stdout
+    for fn in ("benchmarkUpdateGrid(", "launchInit(", "maxPotentialBlockSize()", "copyMask("):
+        assert fn in r.stdout
+    # --shim: the same entry points over libtcsg for kernels loaded as code objects
+    r = _gen("generateLaunchUpdateGrid.py", "Star", "Variable", "4", "3D", "Write", "--shim")
     assert r.returncode == 0 and '#define TCSG_EXPECT_ALIGNMENT "Write"' in r.stdout
     assert _gen("generateLaunchUpdateGrid.py", "Star", "Variable", "4", "4D").returncode == 1
 

@@ -177,3 +177,8 @@ def test_remainder_geometry_limits():
             spec.geometry(d, size)
     t = spec.geometry(d.replace(remainder=False), (2 * r + 5, 2 * r + 3, 2 * r + 9))
     assert t.elements == (4, 2, 8)
+    # PLAIN kernels shift the last thread in x as well
+    p = manifest.cfg4_desc(shape="plain").replace(remainder=True)
+    assert spec.geometry(p, (2 * r + 3, 2 * r + 3, 2 * r + 5)).threads == (2, 2, 2)
+    with pytest.raises(ValueError):
+        spec.geometry(p, (2 * r + 1, 2 * r + 2, 2 * r + 4))

@@ -1,6 +1,7 @@
 """Batched uneven grids on the GPU: the persistent kernel (with and without
 a captured hipGraph of the step loop) and the separate-launch baseline give
-every grid exactly the oracle's result."""
+every grid exactly the oracle's result, also with remainder handling on grid
+sizes that are not rounded to the coarsening."""
 import numpy as np
 import pytest
 
@@ -14,16 +15,22 @@ pytestmark = pytest.mark.gpu
 @pytest.mark.parametrize("dtype", [F64, F32C])
 @pytest.mark.parametrize("mode", ["graph", "launches", "separate"])
 @pytest.mark.parametrize("shape,block,workgroups", [("dppw", (64, 2, 1), 7), ("plain", (32, 4, 2), 7),
-                                                    ("plain", (32, 4, 2), 0), ("plain", (32, 4, 2), -1)])
+                                                    ("plain", (32, 4, 2), 0), ("plain", (32, 4, 2), -1),
+                                                    ("plain+remainder", (32, 4, 2), 0),
+                                                    ("dppw+remainder", (64, 2, 1), 7)])
 def test_batched_matches_oracle(dtype, mode, shape, block, workgroups):
     """workgroups: 7 (fewer than 8: every workgroup strides the whole table),
     0 (as many per CU as fit: the per-XCD ranges), -1 (one per tile)."""
     import torch
 
     from threadcoarsenedstencilgenerator_amd import runtime as rt
+    shape, _, rem = shape.partition("+")
     d = manifest.cfg4_desc(shape=shape, batched=True, dtype=dtype, max_threads=1024 if shape == "plain" else 256)
+    if rem:
+        d = d.replace(remainder=True)
     k = rt.Kernel(d)
-    sizes = rt.batch_sizes(6, 3, 8, 40, d.coarsening, d.range)
+    # remainder kernels take any size: interiors not rounded to the coarsening
+    sizes = rt.batch_sizes(6, 3, 8, 40, (1, 1, 1) if rem else d.coarsening, d.range)
     print(f"batched {mode} {d.key()} sizes {sizes}", flush=True)
     geoms, grids, init = [], [], []
     for i, size in enumerate(sizes):
@@ -52,7 +59,7 @@ def test_batched_matches_oracle(dtype, mode, shape, block, workgroups):
     torch.cuda.synchronize()
     run = oracle.run_f32c if dtype == F32C else oracle.run_f64
     for size, gr, a in zip(sizes, grids, init):
-        og = oracle.Geometry(3, d.range, d.alignment, d.cl_size, size, d.coarsening)
+        og = oracle.Geometry(3, d.range, d.alignment, d.cl_size, size, (1, 1, 1) if rem else d.coarsening)
         ref = run(d, og, None, a.copy(), a.copy(), steps)
         got = gr.host(fin)
         ui = np.uint32 if dtype == F32C else np.uint64

@@ -93,6 +93,50 @@ def build_kernels(descs, force: bool = True, jobs: int | None = None):
     return out
 
 
+# Kernels built the reference's way: kernels.cu (the kernel generator's output) compiled into
+# the launch unit generateLaunchUpdateGrid.py prints, under harness/compiled_in/<name>/, with
+# a harness of our own written against launchUpdateGrid.h (compiled_in_check).
+COMPILED_IN_DIR = os.path.join(PKG, "harness", "compiled_in")
+COMPILED_IN = {
+    "star3d_r4_2x4x128_dppw": dict(dim=3, kernel=["Star", "Row", "Variable", "4", "2", "4", "128", "Write", "16",
+                                                  "--xext=dppw", "--max-threads=256", "--prefetch=0"],
+                                   launch=["Star", "Variable", "4", "3D", "Write"]),
+    "box2d_r1_1x1_plain": dict(dim=2, kernel=["Box", "Row", "Jacobi", "1", "1", "1", "--variant=plain"],
+                               launch=["Box", "Jacobi", "1", "2D"]),
+    "box3d_r2_2x3x2_dpp_read": dict(dim=3, kernel=["Box", "Column", "Variable", "2", "2", "3", "2", "Read", "8",
+                                                   "--xext=dpp"],
+                                    launch=["Box", "Variable", "2", "3D", "Read"]),
+}
+
+
+def compiled_in_desc(name: str) -> spec.StencilDesc:
+    from .generator import cli
+    c = COMPILED_IN[name]
+    return cli.desc_from_argv(c["dim"], c["kernel"])[0]
+
+
+def build_compiled_in(jobs: int | None = None) -> dict:
+    """Writes kernels.cu and launchUpdateGrid.cu for each COMPILED_IN entry
+    and links compiled_in_check against them; returns {name: directory}."""
+    from .generator import cli
+    hd = os.path.join(PKG, "harness")
+
+    def one(name):
+        c = COMPILED_IN[name]
+        d = os.path.join(COMPILED_IN_DIR, name)
+        os.makedirs(d, exist_ok=True)
+        with open(os.path.join(d, "kernels.cu"), "w") as f:
+            f.write(emit_source(compiled_in_desc(name)))
+        with open(os.path.join(d, "launchUpdateGrid.cu"), "w") as f:
+            f.write(cli.launch_unit(c["launch"]))
+        _run([HIPCC, *HIP_FLAGS, "-Wall", f"-I{INCLUDE}", os.path.join(hd, "compiled_in_check.cpp"), "-x", "hip",
+              os.path.join(d, "launchUpdateGrid.cu"), "-o", os.path.join(d, "compiled_in_check")])
+        return d
+
+    with cf.ThreadPoolExecutor(max_workers=jobs or min(4, os.cpu_count() or 1)) as ex:
+        return dict(zip(COMPILED_IN, ex.map(one, COMPILED_IN)))
+
+
 def build_harness() -> str:
     """The `bench` CLI (same argv as the reference's bench) and the
     launchUpdateGrid.h compatibility library."""

@@ -183,9 +183,10 @@ __global__ void error_kernel(const float *__restrict__ a, GridIndex ga, const do
 }
 
 // remainder: keep every point of an axis whose interior is not a multiple of
-// the coarsening (kernels generated for it drop the last thread's columns past
-// the interior in x and start its footprint early in y and z); otherwise the
-// axis is truncated to a multiple, as the reference does.
+// the coarsening (kernels generated for it start the last thread's footprint
+// early enough to end at the interior's end; STREAM kernels in x drop its
+// columns past the interior instead); otherwise the axis is truncated to a
+// multiple, as the reference does.
 int geometry_compute(const tcsg_desc &d, tcsg_dim3 size, tcsg_geometry &g, bool remainder = false)
 {
     g = tcsg_geometry{};
@@ -197,9 +198,10 @@ int geometry_compute(const tcsg_desc &d, tcsg_dim3 size, tcsg_geometry &g, bool 
         if (sz[a] <= 2 * r) return TCSG_ERR_ARG;
         t[a] = remainder ? sz[a] : sz[a] - (sz[a] - 2 * r) % c[a];
         e[a] = t[a] - 2 * r;
-        // remainder kernels shift the last thread in y and z back by less than
-        // one coarsening, which needs an interior of at least one
-        if (remainder && a > 0 && e[a] < c[a]) return TCSG_ERR_ARG;
+        // remainder kernels shift the last thread of an axis back by less than
+        // one coarsening (y and z; x too in PLAIN kernels), which needs an
+        // interior of at least one
+        if (remainder && (a > 0 || d.variant == TCSG_PLAIN) && e[a] < c[a]) return TCSG_ERR_ARG;
     }
     uint64_t xmem = t[0], lo = 0, ro = 0;
     if (d.alignment == TCSG_ALIGN_READ) {
@@ -238,7 +240,7 @@ struct tcsg_kernel {
     bool mask_set = false;
     int legal_max = 0;
     int xext_mode = 0;    // STREAM x-neighbour exchange (params[16]): 0 loads, 1 DPP, 2 wave-aligned DPP
-    bool remainder = false;  // STREAM kernel that masks the last thread's outputs (params[18] bit 2)
+    bool remainder = false;  // kernel generated with remainder handling (params[18] bit 2)
 };
 
 extern "C" {
@@ -322,7 +324,6 @@ int tcsg_kernel_load(tcsg_kernel **out, const char *path, const tcsg_desc *expec
     k->xext_mode = d.variant == TCSG_STREAM ? (int)q[16] : 0;
     if (k->xext_mode > 2) return fail(TCSG_ERR_MISMATCH);
     k->remainder = (q[18] & 4u) != 0;
-    if (k->remainder && d.variant != TCSG_STREAM) return fail(TCSG_ERR_MISMATCH);
     *out = k;
     return TCSG_OK;
 }

@@ -2,7 +2,7 @@
 
     generate2DStencil.py  Box|Star Row|Column Jacobi|Variable range xC yC [None|Read|Write CLsize]
     generate3DStencil.py  Box|Star Row|Column Jacobi|Variable range xC yC zC [None|Read|Write CLsize]
-    generateLaunchUpdateGrid.py  Box|Star Jacobi|Variable range 2D|3D [None|Read|Write]
+    generateLaunchUpdateGrid.py  Box|Star Jacobi|Variable range 2D|3D [None|Read|Write] [--shim]
 
 (hip/generate2DStencil_hip.py:28-60, hip/generate3DStencil_hip.py:29-63,
 hip/generateLaunchUpdateGrid.py:24-45.)  The kernel generators print gfx950
@@ -12,8 +12,8 @@ shape (no reference counterpart) are given as --name=value:
 --variant=plain|stream, --xext=load|dpp|dppw, --dtype=f64|f32c,
 --max-threads=N, --prefetch=N, --waves-per-eu=N, --comp-group=N,
 --order=point|offset (plain variant), --no-xcd-remap, --no-nt-store, --zfirst,
---remainder (stream variant: grids whose interior is not a multiple of the
-coarsening are updated whole instead of truncated).
+--remainder (grids whose interior is not a multiple of the coarsening are
+updated whole instead of truncated).
 Invalid arguments print the usage on stderr and exit with status 1, as the
 reference does.
 """
@@ -106,8 +106,6 @@ def desc_from_argv(dim: int, argv):
             kw[f] = _int(opts[k], k, usage)
     if kw["variant"] == spec.PLAIN:
         kw.update(xext="load", xcd_remap=False, nt_store=False, zfirst=False, order=opts.get("order", "point"))
-        if "remainder" in opts:
-            _err("--remainder applies to the stream variant", usage)
     elif "order" in opts:
         _err("--order applies to the plain variant", usage)
     try:
@@ -133,12 +131,22 @@ def generate(dim: int, argv=None) -> int:
 
 
 def generate_launch(argv=None) -> int:
-    """The reference emits launchUpdateGrid.cu, which compiles the kernels
-    into the harness and checks the parameters baked into them.  Here the
-    kernels are code objects loaded at run time, so this prints a C++ unit
-    that builds the launchUpdateGrid.h entry points over libtcsg with the
-    expected parameters fixed, for harnesses written against that header."""
-    argv = sys.argv[1:] if argv is None else argv
+    """Prints launchUpdateGrid.cu as the reference does
+    (hip/generateLaunchUpdateGrid.py): the four launchUpdateGrid.h entry
+    points over the generated kernels.cu compiled into the same unit, with
+    the expected parameters fixed, so the reference's Makefile builds its
+    harness unchanged.  With --shim it prints instead a C++ unit that builds
+    those entry points over libtcsg, for kernels loaded as code objects at
+    run time."""
+    sys.stdout.write(launch_unit(sys.argv[1:] if argv is None else argv))
+    return 0
+
+
+def launch_unit(argv) -> str:
+    """Text of the launch unit for the reference's generateLaunchUpdateGrid.py
+    argv (plus --shim); invalid arguments exit with status 1 and the usage."""
+    shim = "--shim" in argv
+    argv = [a for a in argv if a != "--shim"]
     if len(argv) not in (4, 5):
         _err("Unknown arguments!", USAGE_L)
     nb, coeff, r, dim = argv[0], argv[1], _int(argv[2], "range", USAGE_L), argv[3]
@@ -154,12 +162,21 @@ def generate_launch(argv=None) -> int:
     if align not in spec.ALIGN_NAMES:
         _err("Unknown alignment!", USAGE_L)
     import os
-    shim = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "harness", "launchUpdateGrid.cpp")
-    print("// Generated by threadcoarsenedstencilgenerator_amd.generator (launch interface).")
-    print(f'#define TCSG_EXPECT_NEIGHBOURHOOD "{nb}"')
-    print(f'#define TCSG_EXPECT_COEFF "{coeff}"')
-    print(f"#define TCSG_EXPECT_RANGE {r}")
-    print(f"#define TCSG_EXPECT_DIMENSION {dim[0]}")
-    print(f'#define TCSG_EXPECT_ALIGNMENT "{align}"')
-    print(f'#include "{shim}"')
-    return 0
+    hdir = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "harness")
+    if not shim:
+        with open(os.path.join(hdir, "compiled_in_launch.cpp.in")) as f:
+            text = f.read()
+        subst = {"@NB@": nb, "@COEFF@": coeff, "@RANGE@": str(r), "@DIM@": dim[0], "@ALIGN@": align,
+                 "@NB_CODE@": str(spec.NB_NAMES[nb]), "@COEFF_CODE@": str(spec.COEFF_NAMES[coeff]),
+                 "@ALIGN_CODE@": str(spec.ALIGN_NAMES[align])}
+        for k, v in subst.items():
+            text = text.replace(k, v)
+        return text
+    shim = os.path.join(hdir, "launchUpdateGrid.cpp")
+    return "\n".join(["// Generated by threadcoarsenedstencilgenerator_amd.generator (launch interface).",
+                      f'#define TCSG_EXPECT_NEIGHBOURHOOD "{nb}"',
+                      f'#define TCSG_EXPECT_COEFF "{coeff}"',
+                      f"#define TCSG_EXPECT_RANGE {r}",
+                      f"#define TCSG_EXPECT_DIMENSION {dim[0]}",
+                      f'#define TCSG_EXPECT_ALIGNMENT "{align}"',
+                      f'#include "{shim}"', ""])

@@ -171,6 +171,9 @@ class _Emitter:
         else:
             w("#include <hip/hip_runtime.h>")
             w("#include <stdint.h>")
+            w("// sums are written with explicit fma; no other product may be fused into an add,")
+            w("// whatever -ffp-contract the including build uses")
+            w("#pragma clang fp contract(off)")
             w("typedef uint32_t u32;")
             w("typedef bool lbool;")
             w(f"typedef {T} TV;")
@@ -184,6 +187,10 @@ class _Emitter:
         w("};")
         blob = ", ".join(str(v) for v in d.params_blob())
         w(f'extern "C" __device__ const uint32_t tcsg_params[24] = {{{blob}}};')
+        if not cpu:
+            # the same block for host code that compiles this source in (a launch unit
+            # that includes it, as the reference's launchUpdateGrid.cu includes kernels.cu)
+            w(f"__attribute__((unused)) static const uint32_t tcsg_params_host[24] = {{{blob}}};")
         w("")
         if not cpu:
             w("template <class V> __device__ __forceinline__ V tcsg_sel(bool c, V a, V b) { return c ? a : b; }")
@@ -367,9 +374,19 @@ class _Emitter:
         # in the persistent batched kernel the body sits in a one-pass loop, so
         # `break` ends this tile where the plain kernel returns
         w(f"  if (!(gx < thx && gy < thy && gz < thz)) {'break' if self.batched_body else 'return'};")
-        w(f"  const size_t x = (size_t)gx * {self.XC} + {R + LO};")
-        w(f"  const size_t y = (size_t)gy * {self.YC} + {R};")
-        w(f"  const size_t z = (size_t)gz * {self.ZC} + {R};" if self.three else "  const size_t z = 0;")
+        if d.remainder:
+            # the last thread of an axis whose interior is not a multiple of the
+            # coarsening starts early enough to end at the interior's end; outputs it
+            # shares with its neighbour get the same bits from both (same inputs, same
+            # order).  The runtime requires interiors of at least one coarsening.
+            w(f"  const size_t x = (size_t)tcsg_min(gx * {self.XC}u, ex - {2 * R + LO + self.XC}u) + {R + LO};")
+            w(f"  const size_t y = (size_t)tcsg_min(gy * {self.YC}u, ydim - {2 * R + self.YC}u) + {R};")
+            w(f"  const size_t z = (size_t)tcsg_min(gz * {self.ZC}u, zdim - {2 * R + self.ZC}u) + {R};"
+              if self.three else "  const size_t z = 0;")
+        else:
+            w(f"  const size_t x = (size_t)gx * {self.XC} + {R + LO};")
+            w(f"  const size_t y = (size_t)gy * {self.YC} + {R};")
+            w(f"  const size_t z = (size_t)gz * {self.ZC} + {R};" if self.three else "  const size_t z = 0;")
         w("  const size_t sx = xdim;")
         w("  const size_t sxy = (size_t)xdim * ydim;")
         w(f"  const {T}* __restrict__ src = a.in + z * sxy + y * sx + x;")

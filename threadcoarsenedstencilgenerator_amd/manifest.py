@@ -66,18 +66,19 @@ def f32c_cases():
 
 
 def remainder_cases():
-    """STREAM kernels with remainder handling on grids whose interiors are
-    not multiples of the coarsening (every parity size is ragged for these
-    coarsenings), in both DPP code shapes, plus FP32C cases."""
+    """Kernels with remainder handling on grids whose interiors are not
+    multiples of the coarsening (every parity size is ragged for these
+    coarsenings): both DPP code shapes and both PLAIN statement orders, plus
+    FP32C cases."""
     out = []
     for d, size in parity_cases():
-        if d.variant != STREAM or d.xext == "load" or d.coarsening == (1, 1, 1):
+        if (d.variant == STREAM and d.xext == "load") or d.coarsening == (1, 1, 1):
             continue
         if d.range == 3 and d.xc == 1:
             continue
         out.append((d.replace(remainder=True), size))
     for d, size in f32c_cases():
-        if d.variant == STREAM:
+        if d.variant == STREAM or d.order == "point":
             out.append((d.replace(remainder=True), (size[0] + 1, size[1] + 2) + tuple(s + 1 for s in size[2:])))
     # four FP32 columns per thread (16-byte row stores) and three FP64 columns
     out.append((StencilDesc(dimension=3, neighbourhood=STAR, orientation=ROW, coeff=VARIABLE, range=2, xc=4, yc=1,
@@ -168,6 +169,9 @@ def default_build_set():
               cfg4_desc(F32C, "plain", batched=True, max_threads=1024)]
     descs += [cfg4_desc(F64, "dppw", batched=True), cfg4_desc(F32C, "dppw", batched=True),
               cfg4_desc(F32C, "plain")]
+    # batched kernels with remainder handling (grids not rounded to the coarsening)
+    descs += [cfg4_desc(dt, "plain", batched=True, max_threads=1024).replace(remainder=True) for dt in (F64, F32C)]
+    descs += [cfg4_desc(dt, "dppw", batched=True).replace(remainder=True) for dt in (F64, F32C)]
     from . import tuned
     descs += tuned.build_set()
     return list(dict.fromkeys(descs))

@@ -66,10 +66,11 @@ class StencilDesc:
     batched: bool = False    # also emit tcsg_update_batched, a persistent kernel over a
     #                          tile table spanning many grids
     nt_load: bool = False    # STREAM: nontemporal hint on the input loads
-    remainder: bool = False  # STREAM: grids whose interior is not a multiple of the coarsening
-    #                          keep every point (the last thread in x drops the columns past
-    #                          the interior, in y and z it starts early enough to end at the
-    #                          interior's end); off, axes are truncated as the reference does
+    remainder: bool = False  # grids whose interior is not a multiple of the coarsening keep
+    #                          every point: the last thread of an axis starts early enough to
+    #                          end at the interior's end (STREAM kernels in x instead drop the
+    #                          columns past the interior); off, axes are truncated as the
+    #                          reference does
     order: str = "point"     # PLAIN statement order: "point" loads each footprint point once and
     #                          adds it to every output that uses it; "offset" walks the mask in
     #                          summation order, every output per offset (weights used in order,
@@ -102,8 +103,6 @@ class StencilDesc:
             raise ValueError("max_threads must be a multiple of 64 in [64, 1024]")
         if self.comp_group < 1:
             raise ValueError("comp_group must be >= 1")
-        if self.remainder and self.variant != STREAM:
-            raise ValueError("remainder handling is a STREAM option")
         if self.order not in ("point", "offset"):
             raise ValueError("order must be 'point' or 'offset'")
         if self.order == "offset" and self.variant != PLAIN:
@@ -278,8 +277,9 @@ def geometry(desc: StencilDesc, size) -> Geometry:
             raise ValueError(f"grid axis {a} ({size[a]}) too small for range {r}")
         t[a] = size[a] if desc.remainder else size[a] - (size[a] - 2 * r) % c[a]
         e[a] = t[a] - 2 * r
-        if desc.remainder and a > 0 and e[a] < c[a]:
-            raise ValueError("remainder handling needs an interior of at least one coarsening in y and z")
+        if desc.remainder and (a > 0 or desc.variant == PLAIN) and e[a] < c[a]:
+            raise ValueError("remainder handling needs an interior of at least one coarsening "
+                             + ("per axis" if desc.variant == PLAIN else "in y and z"))
     lo = ro = 0
     xmem = t[0]
     cl = desc.cl_size

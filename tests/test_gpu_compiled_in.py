@@ -49,6 +49,7 @@ def test_compiled_in_unit_matches_oracle(name, tmp_path):
     print(f"compiled-in {name} size {size} block {block}", flush=True)
     r = subprocess.run(argv, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
+    assert int(r.stdout.split()[1]) <= d.max_threads   # maxPotentialBlockSize within the launch bound
     got = np.fromfile(out, dtype=np.float64)
     og = oracle.Geometry(d.dimension, d.range, d.alignment, d.cl_size, size, d.coarsening)
     assert got.size == og.n_points

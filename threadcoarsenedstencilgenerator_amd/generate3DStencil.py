@@ -3,7 +3,7 @@
 import os
 import sys
 
-sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.realpath(__file__))))
 from threadcoarsenedstencilgenerator_amd.generator.cli import generate  # noqa: E402
 
 if __name__ == "__main__":

@@ -116,14 +116,12 @@ def f32c_companion(k64, g, g64, steps, glups64, stream):
 def bench_batched(args):
     """64 independent 3D Box Jacobi r=2 grids, each interior axis log-uniform
     in [32, 256] (rounded to the 2x2x4 coarsening), 100 time steps.  Modes:
-    one persistent kernel per step (as many workgroups per CU as fit, each
-    XCD's workgroups walking their eighth of the tile table) over a captured
-    hipGraph of the step loop (the headline of this workload), the same
-    without the graph, one workgroup per tile under the graph, and the
-    baseline: 64 separate launches of the plain kernel per step.  The
-    *_unrounded entries run the same draws without the rounding (the kernel
-    generated with remainder handling)."""
-    import numpy as np
+    one persistent kernel per step (as many workgroups per CU as fit, taking
+    tiles from a work queue with one range of the tile table per XCD) over a
+    captured hipGraph of the step loop (the headline of this workload), the
+    same without the graph, and the baseline: 64 separate launches of the
+    plain kernel per step.  The *_unrounded entries run the same draws without
+    the rounding (the kernel generated with remainder handling)."""
     import torch
 
     from threadcoarsenedstencilgenerator_amd import manifest
@@ -131,7 +129,7 @@ def bench_batched(args):
     from threadcoarsenedstencilgenerator_amd.spec import F32C, F64
     out = {}
     steps = 100
-    block = (32, 4, 2)
+    block = (32, 8, 2)
     for name, dt, rem in (("f64", F64, False), ("f32c", F32C, False), ("f64_unrounded", F64, True),
                           ("f32c_unrounded", F32C, True)):
         d = manifest.cfg4_desc(shape="plain", batched=True, dtype=dt, max_threads=1024)
@@ -147,11 +145,9 @@ def bench_batched(args):
                 k.init(g, gr.ptr(j), rt.INIT_U01, 100 + i)
             geoms.append(g)
             grids.append(gr)
-        ntiles = sum(int(np.prod(rt.grid_blocks(g, block))) for g in geoms)
         stream = torch.cuda.current_stream()
         res, info = {}, {}
-        for mode, wg in (("persistent_graph", 0), ("persistent", 0), ("tile_per_workgroup_graph", ntiles),
-                         ("separate_launches", 0)):
+        for mode, wg in (("persistent_graph", 0), ("persistent", 0), ("separate_launches", 0)):
             b = rt.Batch(k, geoms, [g.ptr(0) for g in grids], [g.ptr(1) for g in grids], block, workgroups=wg)
             pts, tiles, nwg = b.info()
             info[mode] = nwg

@@ -179,12 +179,14 @@ int tcsg_time_steps(tcsg_kernel *k, const tcsg_geometry *g, tcsg_dim3 block, tcs
 /* ---- batches of independent grids (kernels generated with the batched
  * option; no reference counterpart) ----
  * One persistent kernel per time step advances every grid of the batch: its
- * workgroups walk a tile table whose entries are (grid, block) pairs, each
- * the work of one block of tcsg_update, the workgroups of each XCD through
- * one contiguous eighth of the table.  geoms[i] is grid i's geometry,
- * bufs0[i]/bufs1[i] its ping-pong buffers; every tile uses `block`;
- * workgroups = 0 launches as many workgroups per compute unit as the
- * kernel's occupancy allows at that block size (at most one per tile). */
+ * workgroups take tiles, (grid, block) pairs each the work of one block of
+ * tcsg_update, from a work queue over the tile table, one contiguous range per
+ * XCD (a workgroup drains its XCD's range, then the others).  geoms[i] is
+ * grid i's geometry, bufs0[i]/bufs1[i] its ping-pong buffers; every tile uses
+ * `block`; workgroups = 0 launches as many workgroups per compute unit as the
+ * kernel's occupancy allows at that block size (at most one per tile).  The
+ * queue is the batch's own device memory, cleared by each launch's last
+ * workgroup: runs of one batch must not overlap in time. */
 typedef struct tcsg_batch tcsg_batch;
 int tcsg_batch_create(tcsg_batch **out, tcsg_kernel *k, uint32_t n, const tcsg_geometry *geoms,
                       void *const *bufs0, void *const *bufs1, tcsg_dim3 block, uint32_t workgroups);

@@ -603,8 +603,10 @@ struct BatchArgs {
     const KArgs *grids;
     const Tile *tiles;
     uint32_t ntiles, pad;
+    uint32_t *queue;  // the kernel's work queue: 8 per-XCD range counters + finished workgroups
 };
-static_assert(sizeof(BatchArgs) == 24, "batch argument layout");
+static_assert(sizeof(BatchArgs) == 32, "batch argument layout");
+constexpr size_t kQueueWords = 16;
 } // namespace
 
 struct tcsg_batch {
@@ -618,6 +620,7 @@ struct tcsg_batch {
     uint64_t points = 0;
     KArgs *d_grids[2] = {nullptr, nullptr};
     Tile *d_tiles = nullptr;
+    uint32_t *d_queue = nullptr;  // zero between launches (the kernel's last workgroup clears it)
     hipStream_t own = nullptr;
     std::vector<std::pair<int, hipGraphExec_t>> graphs;
 };
@@ -631,6 +634,7 @@ int tcsg_batch_free(tcsg_batch *b)
     for (auto *p : b->d_grids)
         if (p) (void)hipFree(p);
     if (b->d_tiles) (void)hipFree(b->d_tiles);
+    if (b->d_queue) (void)hipFree(b->d_queue);
     if (b->own) (void)hipStreamDestroy(b->own);
     delete b;
     return TCSG_OK;
@@ -703,6 +707,9 @@ int tcsg_batch_create(tcsg_batch **out, tcsg_kernel *k, uint32_t n, const tcsg_g
     }
     if (e == hipSuccess) e = hipMalloc(&b->d_tiles, tiles.size() * sizeof(Tile));
     if (e == hipSuccess) e = hipMemcpy(b->d_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&b->d_queue, kQueueWords * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemset(b->d_queue, 0, kQueueWords * sizeof(uint32_t));  // before any capture
+    if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->own, hipStreamNonBlocking);
     if (e != hipSuccess) {
         g_last_hip = (int)e;
@@ -715,7 +722,7 @@ int tcsg_batch_create(tcsg_batch **out, tcsg_kernel *k, uint32_t n, const tcsg_g
 
 static int batch_launch(tcsg_batch *b, int parity, hipStream_t s)
 {
-    BatchArgs a{b->d_grids[parity], b->d_tiles, b->ntiles, 0};
+    BatchArgs a{b->d_grids[parity], b->d_tiles, b->ntiles, 0, b->d_queue};
     size_t sz = sizeof(a);
     void *extra[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, &a, HIP_LAUNCH_PARAM_BUFFER_SIZE, &sz, HIP_LAUNCH_PARAM_END};
     TCSG_HIP(hipModuleLaunchKernel(b->fn, b->nwg, 1, 1, b->block.x, b->block.y, b->block.z, 0, s, nullptr, extra));

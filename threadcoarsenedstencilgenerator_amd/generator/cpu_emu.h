@@ -16,6 +16,7 @@
 #include <cstdint>
 #include <cstring>
 #include <type_traits>
+#include <vector>
 
 #define __global__
 #define __device__

@@ -300,21 +300,32 @@ class _Emitter:
         w("  return 0;")
         w("}")
         if self.d.batched:
+            w("// One launch of the batched kernel: the workgroups take turns (round robin) to")
+            w("// take a tile from the queue and run every wave on it; a workgroup that finds")
+            w("// the queue drained finishes as the GPU kernel's does.")
             w('extern "C" int tcsg_emulate_batched(const void* grids, const void* tiles, uint32_t ntiles,')
-            w("                                    uint32_t nwg, uint32_t bx, uint32_t by, uint32_t bz) {")
+            w("                                    uint32_t* queue, uint32_t nwg, uint32_t bx, uint32_t by, uint32_t bz) {")
             w("  const uint32_t nt = bx * by * bz;")
-            w("  if (nt == 0 || nt % 64) return -1;")
+            w("  if (nt == 0 || nt % 64 || nwg == 0) return -1;")
             w("  gridDim = {nwg, 1, 1};")
             w("  blockDim = {bx, by, bz};")
-            w("  tcsg_batch_args ba{(const tcsg_kargs*)grids, (const tcsg_tile*)tiles, ntiles, 0};")
-            w("  for (uint32_t i = 0; i < nwg; ++i) {")
-            w("    blockIdx = {i, 0, 0};")
-            w("    for (uint32_t w0 = 0; w0 < nt; w0 += 64) {")
-            w("      for (int l = 0; l < 64; ++l) {")
-            w("        const uint32_t lid = w0 + l;")
-            w("        threadIdx.x.v[l] = lid % bx; threadIdx.y.v[l] = (lid / bx) % by; threadIdx.z.v[l] = lid / (bx * by);")
+            w("  tcsg_batch_args ba{(const tcsg_kargs*)grids, (const tcsg_tile*)tiles, ntiles, 0, queue};")
+            w("  const uint32_t nr = nwg >= 8u ? 8u : 1u;")
+            w("  std::vector<uint32_t> s(nwg, 0u);")
+            w("  std::vector<char> live(nwg, 1);")
+            w("  for (uint32_t nlive = nwg; nlive;) {")
+            w("    for (uint32_t i = 0; i < nwg; ++i) {")
+            w("      if (!live[i]) continue;")
+            w("      const uint32_t t = tcsg_fetch(queue, ntiles, nr == 8u ? (i & 7u) : 0u, nr, s[i]);")
+            w("      if (t == 0xFFFFFFFFu) { tcsg_finish(queue, nwg); live[i] = 0; --nlive; continue; }")
+            w("      blockIdx = {i, 0, 0};")
+            w("      for (uint32_t w0 = 0; w0 < nt; w0 += 64) {")
+            w("        for (int l = 0; l < 64; ++l) {")
+            w("          const uint32_t lid = w0 + l;")
+            w("          threadIdx.x.v[l] = lid % bx; threadIdx.y.v[l] = (lid / bx) % by; threadIdx.z.v[l] = lid / (bx * by);")
+            w("        }")
+            w("        tcsg_update_batched_tile(ba, t);")
             w("      }")
-            w("      tcsg_update_batched(ba);")
             w("    }")
             w("  }")
             w("  return 0;")
@@ -707,30 +718,78 @@ class _Emitter:
         return "\n".join(self.L) + "\n"
 
     def batched_kernel(self, attr):
-        """Persistent kernel: the workgroups stride over a tile table whose
-        entries name a grid (its kernel arguments) and a block of that grid;
-        each tile is the work of one block of tcsg_update."""
+        """Persistent kernel: the workgroups take tiles from a work queue over a
+        tile table whose entries name a grid (its kernel arguments) and a block
+        of that grid; each tile is the work of one block of tcsg_update."""
         w = self.w
         d = self.d
+        cpu = self.target == "cpu"
         w("")
         w("struct tcsg_tile { uint32_t grid, bx, by, bz; };")
-        w("struct tcsg_batch_args { const tcsg_kargs* grids; const tcsg_tile* tiles; uint32_t ntiles, pad; };")
-        w(f'extern "C" __global__ void __launch_bounds__({d.max_threads}){attr} '
-          f"tcsg_update_batched(const tcsg_batch_args ba)")
-        w("{")
-        w("  // XCD-aware tile ranges: the hardware hands workgroup i to XCD i % 8, so the")
-        w("  // workgroups of one XCD walk one contiguous eighth of the tile table (whose")
-        w("  // neighbouring entries are neighbouring blocks of one grid, sharing halo lines")
-        w("  // in that XCD's L2).  With fewer than 8 workgroups all of them stride the table.")
-        w("  const uint32_t nwg = gridDim.x;")
-        w("  uint32_t t0 = blockIdx.x, tend = ba.ntiles, tstep = nwg;")
-        w("  if (nwg >= 8u) {")
-        w("    const uint32_t xcd = blockIdx.x & 7u;")
-        w("    t0 = (uint32_t)(((uint64_t)ba.ntiles * xcd) >> 3) + (blockIdx.x >> 3);")
-        w("    tend = (uint32_t)(((uint64_t)ba.ntiles * (xcd + 1u)) >> 3);")
-        w("    tstep = (nwg - xcd + 7u) >> 3;  // workgroups on this XCD")
+        w("struct tcsg_batch_args { const tcsg_kargs* grids; const tcsg_tile* tiles; uint32_t ntiles, pad;")
+        w("                         uint32_t* queue; };")
+        w("// Work queue: the tile table is split into one contiguous range per XCD (the")
+        w("// hardware hands workgroup i to XCD i % 8; neighbouring entries are neighbouring")
+        w("// blocks of one grid, sharing halo lines in that XCD's L2), or a single range with")
+        w("// fewer than 8 workgroups.  queue[x] counts the tiles taken from range x;")
+        w("// queue[8] the workgroups that found every range drained.  A workgroup drains its")
+        w("// own range, then takes from the following ones.  The last workgroup to finish")
+        w("// clears the queue, so each launch (and each replay of a captured step loop)")
+        w("// starts from zero; launches of one batch must not overlap.")
+        if cpu:
+            HD = "static inline"
+            w("static inline uint32_t tcsg_atomic_add(uint32_t* p, uint32_t v) { const uint32_t o = *p; *p = o + v; return o; }")
+            w("static inline void tcsg_atomic_store(uint32_t* p, uint32_t v) { *p = v; }")
+            w("static inline void tcsg_fence() {}")
+        else:
+            HD = "__device__ __forceinline__"
+            w("__device__ __forceinline__ uint32_t tcsg_atomic_add(uint32_t* p, uint32_t v) { return atomicAdd(p, v); }")
+            w("__device__ __forceinline__ void tcsg_atomic_store(uint32_t* p, uint32_t v) { (void)atomicExch(p, v); }")
+            w("__device__ __forceinline__ void tcsg_fence() { __threadfence(); }")
+        w("// next tile for a workgroup on XCD xcd of nr ranges, or ~0u when all are drained;")
+        w("// s: ranges (counted from xcd) this workgroup has found drained")
+        w(f"{HD} uint32_t tcsg_fetch(uint32_t* q, uint32_t ntiles, uint32_t xcd, uint32_t nr, uint32_t& s) {{")
+        w("  for (; s < nr; ++s) {")
+        w("    const uint32_t x = (xcd + s) % nr;")
+        w("    const uint32_t beg = (uint32_t)(((uint64_t)ntiles * x) / nr);")
+        w("    const uint32_t end = (uint32_t)(((uint64_t)ntiles * (x + 1u)) / nr);")
+        w("    if (beg == end) continue;")
+        w("    const uint32_t k = tcsg_atomic_add(&q[x], 1u);")
+        w("    if (k < end - beg) return beg + k;")
         w("  }")
-        w("  for (uint32_t t = t0; t < tend; t += tstep) {")
+        w("  return 0xFFFFFFFFu;")
+        w("}")
+        w(f"{HD} void tcsg_finish(uint32_t* q, uint32_t nwg) {{")
+        w("  tcsg_fence();")
+        w("  // the last workgroup here: every other one is done with the queue")
+        w("  if (tcsg_atomic_add(&q[8], 1u) == nwg - 1u) {")
+        w("    for (uint32_t x = 0; x < 8u; ++x) tcsg_atomic_store(&q[x], 0u);")
+        w("    tcsg_atomic_store(&q[8], 0u);")
+        w("  }")
+        w("}")
+        if cpu:
+            # the emulation runs one wave at a time: the runner takes the tiles from the
+            # queue (tcsg_fetch / tcsg_finish above) and runs every wave of the workgroup
+            # on each, where the GPU kernel hands the tile to its waves through LDS
+            w("void tcsg_update_batched_tile(const tcsg_batch_args ba, const uint32_t t)")
+            w("{")
+        else:
+            w(f'extern "C" __global__ void __launch_bounds__({d.max_threads}){attr} '
+              f"tcsg_update_batched(const tcsg_batch_args ba)")
+            w("{")
+            w("  __shared__ uint32_t tcsg_slot[2];  // the tile, from the leader to every wave")
+            w("  const uint32_t nwg = gridDim.x;")
+            w("  const uint32_t nr = nwg >= 8u ? 8u : 1u;")
+            w("  const uint32_t xcd = nr == 8u ? (blockIdx.x & 7u) : 0u;")
+            w("  const bool leader = threadIdx.x == 0 && threadIdx.y == 0 && threadIdx.z == 0;")
+            w("  uint32_t s = 0;")
+            w("  for (uint32_t it = 0;; ++it) {")
+            w("  // alternate slots: the leader writes the next one only after every wave has")
+            w("  // passed this barrier, i.e. has read the previous one")
+            w("  if (leader) tcsg_slot[it & 1u] = tcsg_fetch(ba.queue, ba.ntiles, xcd, nr, s);")
+            w("  __syncthreads();")
+            w("  const uint32_t t = tcsg_slot[it & 1u];")
+            w("  if (t == 0xFFFFFFFFu) break;")
         w("  const tcsg_tile tl = ba.tiles[t];")
         w("  const tcsg_kargs a = ba.grids[tl.grid];")
         self.batched_body = True
@@ -741,7 +800,9 @@ class _Emitter:
         else:
             self.body_stream()
         self.batched_body = False
-        w("  }")
+        if not cpu:
+            w("  }")
+            w("  if (leader) tcsg_finish(ba.queue, nwg);")
         w("}")
 
 

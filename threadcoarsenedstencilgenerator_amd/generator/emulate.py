@@ -70,8 +70,8 @@ def run_batched(desc: spec.StencilDesc, geoms, arrays0, arrays1, mask, steps: in
     """Emulate tcsg_update_batched over several grids (tile table built as the
     runtime builds it: grid-major, x fastest).  Returns (final arrays, bad)."""
     L = C.CDLL(build(desc))
-    L.tcsg_emulate_batched.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
-                                       C.c_uint32]
+    L.tcsg_emulate_batched.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32,
+                                       C.c_uint32, C.c_uint32]
     dt = np.float32 if desc.dtype == F32C else np.float64
     m = np.ascontiguousarray(mask if mask is not None else np.zeros(1), dtype=dt)
     tiles = []
@@ -83,13 +83,17 @@ def run_batched(desc: spec.StencilDesc, geoms, arrays0, arrays1, mask, steps: in
                     tiles.append((gi, x, y, z))
     tiles = np.array(tiles, dtype=np.uint32)
     bufs = [list(arrays0), list(arrays1)]
+    queue = np.zeros(16, dtype=np.uint32)   # the kernel's work queue, zero before the first launch
     for s in range(steps):
         ka = (_KArgs * len(geoms))()
         for i, g in enumerate(geoms):
             src, dst = bufs[s & 1][i], bufs[(s + 1) & 1][i]
             ka[i] = _KArgs(src.ctypes.data, dst.ctypes.data, m.ctypes.data, g.xdim, g.ydim, g.zdim, g.ro,
                            src.nbytes, dst.nbytes, 0, 0)
-        if L.tcsg_emulate_batched(C.cast(ka, C.c_void_p), tiles.ctypes.data, len(tiles), nwg, *block) != 0:
+        if L.tcsg_emulate_batched(C.cast(ka, C.c_void_p), tiles.ctypes.data, len(tiles), queue.ctypes.data, nwg,
+                                  *block) != 0:
             raise ValueError("block shape must be whole waves")
+        if queue.any():
+            raise AssertionError(f"work queue not cleared by the launch's last workgroup: {queue}")
     bad = C.c_ulong.in_dll(L, "tcsg_emu_bad_access").value
     return bufs[steps & 1], bad

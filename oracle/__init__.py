@@ -41,7 +41,7 @@ def build(ref: bool = True) -> None:
     reference's own primes.cpp, and its harness over the compiled-in launch
     units the package build wrote, into oracle/_ref)."""
     for target in ("all", "ref") if ref and os.path.isdir("/root/reference/hip") else ("all",):
-        r = subprocess.run(["make", "-s", "-C", HERE, target], capture_output=True, text=True)
+        r = subprocess.run(["make", "-s", "-j4", "-C", HERE, target], capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"make {target} in {HERE} failed:\n{r.stdout}\n{r.stderr}")
 

@@ -21,6 +21,18 @@ def _gen(script, *args):
 def test_generator_argv_like_reference():
     r = _gen("generate2DStencil.py", "Box", "Row", "Jacobi", "1", "1", "1")
     assert r.returncode == 0 and "tcsg_update" in r.stdout
+    # the reference kernel file's symbols (hip/generate2DStencil_hip.py:70-76,221-226,322-330)
+    assert ("__global__ void __launch_bounds__(1024) updateGrid(const unsigned long xdim, const unsigned long ydim, "
+            "const double * const __restrict__ array0, double * const __restrict__ array1)") in r.stdout
+    assert "#define range 1" in r.stdout and "__global__ void init(double * array, const unsigned long xdim, " \
+        "const unsigned long ydim)" in r.stdout and "d_mask" not in r.stdout
+    r = _gen("generate3DStencil.py", "Star", "Row", "Variable", "4", "1", "2", "8", "Write", "16")
+    assert ("updateGrid(const unsigned long xdim, const unsigned long ydim, const unsigned long zdim, "
+            "double * const __restrict__ array0, double * const __restrict__ array1, "
+            "const unsigned int rightOffset)") in r.stdout
+    assert "__constant__ double d_mask[25];" in r.stdout and "+ 12;" in r.stdout
+    r = _gen("generate3DStencil.py", "Star", "Row", "Variable", "4", "1", "2", "8", "--dtype=f32c")
+    assert r.returncode == 0 and "updateGrid" not in r.stdout   # FP32 storage: no FP64 interface
     r = _gen("generate3DStencil.py", "Star", "Column", "Variable", "4", "1", "2", "8", "Write", "16")
     assert r.returncode == 0 and "tcsg_params[24] = {1413698375, 1, 3, 1, 1, 1, 4, 1, 2, 8, 2, 16," in r.stdout
     # wrong argument counts / values: usage on stderr, status 1 (hip/generate2DStencil_hip.py:28-60)

@@ -2,10 +2,14 @@
 into the launch unit that generateLaunchUpdateGrid.py prints (as
 hip/generateLaunchUpdateGrid.py:55-58 compiles its kernels.cu in), driven
 
-- by a harness of our own written against launchUpdateGrid.h
-  (harness/compiled_in_check.cpp), set up as hip/benchmark.cpp sets up its
-  grids (the reference's initial values, weights 1/n): bit for bit against the
-  oracle after three steps; and
+- through that unit's launchUpdateGrid.h entry points, set up as
+  hip/benchmark.cpp sets up its grids (the reference's initial values, weights
+  1/n), by a harness of our own (harness/compiled_in_check.cpp, mode "unit"):
+  bit for bit against the oracle after three steps;
+- through the kernel file's own symbols (updateGrid, init, d_mask), launched
+  the way the reference's generated unit launches them (mode "plugin"): bit for
+  bit as well, also with a block shape the fast body cannot run, which takes
+  the reference-shaped body; and
 - by the reference's own harness (hip/main.cpp, benchmark.cpp, primes.cpp,
   compiled from its sources into oracle/_ref by oracle/Makefile), whose
   block-shape sweep runs the generated kernel and prints its TSV.
@@ -33,23 +37,29 @@ COEFFS = {spec.JACOBI: "Jacobi", spec.VARIABLE: "Variable"}
 ALIGNS = ["None", "Read", "Write"]
 
 
-def _check_args(name, size, block, steps, out):
+def _check_args(name, size, block, steps, out, mode="unit"):
     d = build.compiled_in_desc(name)
     g = spec.geometry(d, size)
-    return d, g, [os.path.join(build.COMPILED_IN_DIR, name, "compiled_in_check"), NAMES[d.neighbourhood],
+    return d, g, [os.path.join(build.COMPILED_IN_DIR, name, "compiled_in_check"), mode, NAMES[d.neighbourhood],
                   COEFFS[d.coeff], str(d.range), str(d.dimension), ALIGNS[d.alignment], str(g.xdim), str(g.ydim),
                   str(g.zdim), str(g.ro), *map(str, block), *map(str, g.threads), str(steps), out]
 
 
-@pytest.mark.parametrize("name", list(CASES))
-def test_compiled_in_unit_matches_oracle(name, tmp_path):
-    size, block = CASES[name]
+RUNS = [(name, "unit", CASES[name][1]) for name in CASES] + [(name, "plugin", CASES[name][1]) for name in CASES]
+# the wave-aligned DPP body needs block rows of whole waves: updateGrid takes the reference-shaped body
+RUNS.append(("star3d_r4_2x4x128_dppw", "plugin", (96, 2, 1)))
+
+
+@pytest.mark.parametrize("name,mode,block", RUNS)
+def test_compiled_in_kernel_matches_oracle(name, mode, block, tmp_path):
+    size, _ = CASES[name]
     out = str(tmp_path / "grid.bin")
-    d, g, argv = _check_args(name, size, block, 3, out)
-    print(f"compiled-in {name} size {size} block {block}", flush=True)
+    d, g, argv = _check_args(name, size, block, 3, out, mode)
+    print(f"compiled-in {name} {mode} size {size} block {block}", flush=True)
     r = subprocess.run(argv, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
-    assert int(r.stdout.split()[1]) <= d.max_threads   # maxPotentialBlockSize within the launch bound
+    if mode == "unit":
+        assert int(r.stdout.split()[1]) <= d.max_threads   # maxPotentialBlockSize within the launch bound
     got = np.fromfile(out, dtype=np.float64)
     og = oracle.Geometry(d.dimension, d.range, d.alignment, d.cl_size, size, d.coarsening)
     assert got.size == og.n_points

@@ -93,9 +93,10 @@ def build_kernels(descs, force: bool = True, jobs: int | None = None):
     return out
 
 
-# Kernels built the reference's way: kernels.cu (the kernel generator's output) compiled into
-# the launch unit generateLaunchUpdateGrid.py prints, under harness/compiled_in/<name>/, with
-# a harness of our own written against launchUpdateGrid.h (compiled_in_check).
+# Kernels built the reference's way: kernels.cu (the kernel generator's output, with the
+# reference kernel file's symbols) compiled into the launch unit generateLaunchUpdateGrid.py
+# prints, under harness/compiled_in/<name>/, with a check harness of our own that drives both
+# the unit's launchUpdateGrid.h entry points and the kernel file's symbols (compiled_in_check).
 COMPILED_IN_DIR = os.path.join(PKG, "harness", "compiled_in")
 COMPILED_IN = {
     "star3d_r4_2x4x128_dppw": dict(dim=3, kernel=["Star", "Row", "Variable", "4", "2", "4", "128", "Write", "16",
@@ -117,7 +118,7 @@ def compiled_in_desc(name: str) -> spec.StencilDesc:
 
 def build_compiled_in(jobs: int | None = None) -> dict:
     """Writes kernels.cu and launchUpdateGrid.cu for each COMPILED_IN entry
-    and links compiled_in_check against them; returns {name: directory}."""
+    and compiles compiled_in_check with them; returns {name: directory}."""
     from .generator import cli
     hd = os.path.join(PKG, "harness")
 
@@ -126,11 +127,11 @@ def build_compiled_in(jobs: int | None = None) -> dict:
         d = os.path.join(COMPILED_IN_DIR, name)
         os.makedirs(d, exist_ok=True)
         with open(os.path.join(d, "kernels.cu"), "w") as f:
-            f.write(emit_source(compiled_in_desc(name)))
+            f.write(emit_source(compiled_in_desc(name), plugin=True))
         with open(os.path.join(d, "launchUpdateGrid.cu"), "w") as f:
             f.write(cli.launch_unit(c["launch"]))
-        _run([HIPCC, *HIP_FLAGS, "-Wall", f"-I{INCLUDE}", os.path.join(hd, "compiled_in_check.cpp"), "-x", "hip",
-              os.path.join(d, "launchUpdateGrid.cu"), "-o", os.path.join(d, "compiled_in_check")])
+        _run([HIPCC, *HIP_FLAGS, "-Wall", f"-I{INCLUDE}", f"-I{d}", os.path.join(hd, "compiled_in_check.cpp"),
+              "-o", os.path.join(d, "compiled_in_check")])
         return d
 
     with cf.ThreadPoolExecutor(max_workers=jobs or min(4, os.cpu_count() or 1)) as ex:

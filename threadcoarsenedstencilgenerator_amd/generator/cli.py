@@ -126,7 +126,8 @@ def generate(dim: int, argv=None) -> int:
             open(src, "w").write(emit_source(d))
             _run([HIPCC, *HIP_FLAGS, "--genco", "--no-gpu-bundle-output", src, "-o", out])
     else:
-        sys.stdout.write(emit_source(d))
+        # the printed kernels.cu also carries the reference kernel file's symbols (FP64)
+        sys.stdout.write(emit_source(d, plugin=d.dtype == spec.F64))
     return 0
 
 

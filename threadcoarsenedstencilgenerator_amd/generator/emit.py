@@ -44,7 +44,7 @@ from __future__ import annotations
 from collections import defaultdict
 
 from .. import spec
-from ..spec import (ALIGN_NONE, BOX, COLUMN, F32C, F64, JACOBI, PLAIN, ROW, STAR,
+from ..spec import (ALIGN_NONE, ALIGN_WRITE, BOX, COLUMN, F32C, F64, JACOBI, PLAIN, ROW, STAR,
                     STREAM, VARIABLE, StencilDesc)
 
 # Byte offset given to lanes outside the interior.  The runtime only launches a
@@ -698,7 +698,7 @@ class _Emitter:
                             w(f"  tcsg_st(rout, spart{c}, {soff(j, kk)}, {row[c]});")
 
     # ------------------------------------------------------------------ all
-    def source(self) -> str:
+    def source(self, plugin: bool = False) -> str:
         d = self.d
         self.preamble()
         attr = ""
@@ -713,9 +713,121 @@ class _Emitter:
         self.w("}")
         if d.batched:
             self.batched_kernel(attr)
+        if plugin:
+            self.reference_entry_points(attr)
         if self.target == "cpu":
             self.emulation_runner()
         return "\n".join(self.L) + "\n"
+
+    def reference_entry_points(self, attr):
+        """The symbols the reference's kernel file provides to the launch unit
+        that its generator prints (hip/generateLaunchUpdateGrid.py:55-58):
+        updateGrid with the reference's arguments (hip/generate2DStencil_hip.py:70-76,
+        hip/generate3DStencil_hip.py:73-79), init with its initial values
+        (hip/generate2DStencil_hip.py:322-351, hip/generate3DStencil_hip.py:387-431),
+        the __constant__ weights d_mask and the macro range (:221-226).  That
+        launch unit checks nothing, so updateGrid runs this kernel's body only
+        where it applies (block shape, 32-bit buffer addressing) and the
+        reference's code shape, which sums in the same order, elsewhere."""
+        d = self.d
+        w = self.w
+        R, three = d.range, self.three
+        aligned = d.alignment != ALIGN_NONE
+        w("")
+        w("// ---- the reference kernel file's symbols, for its own launchUpdateGrid.cu ----")
+        w(f"#define TCSG_PLUGIN_DIM {d.dimension}")
+        w(f"#define TCSG_PLUGIN_RIGHT_OFFSET {int(aligned)}")
+        w(f"#define TCSG_PLUGIN_VARIABLE {int(d.coeff == VARIABLE)}")
+        w(f"#define TCSG_PLUGIN_STENCIL_SIZE {d.n}")
+        w("#ifndef range")
+        w(f"#define range {R}")
+        w("#endif")
+        if d.coeff == VARIABLE:
+            w(f"__constant__ double d_mask[{d.n}];")
+
+        def device_body(name, desc):
+            e = _Emitter(desc, self.target)
+            if desc.variant == PLAIN:
+                e.body_plain()
+            else:
+                e.body_stream()
+            w(f"__device__ __forceinline__ void {name}(const tcsg_kargs a)")
+            w("{")
+            self.L.extend(e.L)
+            w("}")
+
+        device_body("tcsg_step_fast", d)
+        fallback = d.variant == STREAM
+        if fallback:
+            device_body("tcsg_step_plain", d.replace(variant=PLAIN, xext="load", xcd_remap=False, nt_store=False,
+                                                     nt_load=False, zfirst=False, prefetch=0, waves_per_eu=0,
+                                                     batched=False, order="point"))
+        args = "const unsigned long xdim, const unsigned long ydim"
+        if three:
+            args += ", const unsigned long zdim, double * const __restrict__ array0"
+        else:
+            args += ", const double * const __restrict__ array0"
+        args += ", double * const __restrict__ array1"
+        if aligned:
+            args += ", const unsigned int rightOffset"
+        w(f"__global__ void __launch_bounds__({d.max_threads}){attr} updateGrid({args})")
+        w("{")
+        w("  tcsg_kargs a;")
+        w("  a.in = array0;")
+        w("  a.out = array1;")
+        w(f"  a.mask = {'d_mask' if d.coeff == VARIABLE else 'nullptr'};")
+        w("  a.xdim = (uint32_t)xdim;")
+        w("  a.ydim = (uint32_t)ydim;")
+        w(f"  a.zdim = {'(uint32_t)zdim' if three else '1u'};")
+        w(f"  a.right_offset = {'rightOffset' if aligned else '0u'};")
+        w(f"  const uint64_t bytes = (uint64_t)xdim * ydim{' * zdim' if three else ''} * 8u;")
+        w("  a.in_bytes = a.out_bytes = (uint32_t)(bytes < 0xFFFFFFFFull ? bytes : 0xFFFFFFFFull);")
+        w("  a.pad0 = a.pad1 = 0u;")
+        if fallback:
+            H = self.halo_lanes()
+            w("  // this kernel's body needs buffers within its 32-bit addressing (lanes outside the")
+            w("  // interior are sent 3 GiB past the buffer start) and, for its x-neighbour exchange,")
+            w("  // the block shapes the runtime's tcsg_check_block accepts; the reference's code")
+            w("  // shape, summing in the same order, takes the launches that do not fit")
+            w("  const uint64_t sx = (uint64_t)xdim * 8u, sxy = sx * ydim;")
+            w(f"  const uint64_t span = {d.zc + 2 * R}ull * {'sxy' if three else '0ull'} + {d.yc + 2 * R}ull * sx + "
+              f"{(2 * H * d.xc + d.xc + 2 * R) * 8}ull;")
+            w(f"  bool fast = bytes + span <= {OOB_BASE:#x}ull;")
+            if d.xext == "dppw":
+                w("  fast = fast && blockDim.x % 64u == 0u;")
+            elif d.xext == "dpp":
+                w("  {")
+                w("    // every run of lanes in one block row and one wave spans the lanes the exchange reaches")
+                w("    const uint32_t nt = blockDim.x * blockDim.y * blockDim.z;")
+                w("    const uint32_t lid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);")
+                w("    const uint32_t lane = lid & 63u, tx = threadIdx.x;")
+                w("    const uint32_t tl = tx < lane ? tx : lane;")
+                w("    uint32_t tr = blockDim.x - 1u - tx < 63u - lane ? blockDim.x - 1u - tx : 63u - lane;")
+                w("    tr = tr < nt - 1u - lid ? tr : nt - 1u - lid;")
+                w(f"    fast = __syncthreads_and(fast && tl + tr + 1u >= {H}u) != 0;")
+                w("  }")
+            w("  if (fast) tcsg_step_fast(a); else tcsg_step_plain(a);")
+        else:
+            w("  tcsg_step_fast(a);")
+        w("}")
+        lo = d.left_offset if d.alignment == ALIGN_WRITE else 0
+        iargs = "double * array, const unsigned long xdim, const unsigned long ydim"
+        if three:
+            iargs += ", const unsigned long zdim"
+        if aligned:
+            iargs += ", const unsigned int rightOffset"
+        w(f"__global__ void init({iargs})")
+        w("{")
+        w(f"  const size_t x = blockIdx.x * blockDim.x + threadIdx.x + {lo};")
+        w("  const size_t y = blockIdx.y * blockDim.y + threadIdx.y;")
+        if three:
+            w("  const size_t z = blockIdx.z * blockDim.z + threadIdx.z;")
+        w(f"  if (x < xdim{' - rightOffset' if aligned else ''} && y < ydim{' && z < zdim' if three else ''}) {{")
+        w(f"    const size_t idx = x + y * xdim{' + z * xdim * ydim' if three else ''};")
+        w(f"    const bool halo = x - {lo} < range || y < range{' || z < range' if three else ''};")
+        w(f"    array[idx] = halo ? {'(idx % 2 == 0 ? 0.75 : 0.25)' if three else '1.0'} : 0.0;")
+        w("  }")
+        w("}")
 
     def batched_kernel(self, attr):
         """Persistent kernel: the workgroups take tiles from a work queue over a
@@ -816,10 +928,15 @@ def terms_per_thread(desc: StencilDesc) -> int:
     return desc.xc * desc.yc * desc.zc * desc.n
 
 
-def emit_source(desc: StencilDesc, target: str = "gfx950") -> str:
+def emit_source(desc: StencilDesc, target: str = "gfx950", plugin: bool = False) -> str:
     """Source text of the kernel for ``desc``: HIP for gfx950, or (target
     "cpu", STREAM variant only) the same kernel body over the wave-level CPU
-    emulation in cpu_emu.h, used to test the emitted code on the host."""
+    emulation in cpu_emu.h, used to test the emitted code on the host.
+    plugin (FP64 kernels for gfx950): also define the reference kernel file's
+    updateGrid, init, d_mask and range, so the source can stand in for the
+    reference's kernels.cu under the launch unit its generator prints."""
+    if plugin and (target != "gfx950" or desc.dtype != F64):
+        raise ValueError("the reference's kernel-file symbols exist for FP64 kernels built for the GPU")
     if terms_per_thread(desc) > MAX_TERMS_PER_THREAD:
         raise ValueError(f"coarsening {desc.coarsening} x stencil size {desc.n} = {terms_per_thread(desc)} "
                          f"accumulate statements per thread exceeds {MAX_TERMS_PER_THREAD}")
@@ -827,4 +944,4 @@ def emit_source(desc: StencilDesc, target: str = "gfx950") -> str:
         raise ValueError("CPU emulation covers the STREAM variant")
     if target not in ("gfx950", "cpu"):
         raise ValueError(f"unknown target {target}")
-    return _Emitter(desc, target).source()
+    return _Emitter(desc, target).source(plugin)

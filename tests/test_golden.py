@@ -2,7 +2,6 @@
 the oracle computes today, for every pinned run; tests/golden/make_oracle_golden.py
 rewrites them deliberately."""
 import json
-import os
 
 from .golden.make_oracle_golden import OUT, compute
 

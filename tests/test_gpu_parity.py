@@ -4,7 +4,7 @@ alignment) in the three code shapes (plain, stream, stream+DPP), on small
 grids with truncated and padded geometry and a few time steps."""
 import pytest
 
-from threadcoarsenedstencilgenerator_amd import manifest, spec
+from threadcoarsenedstencilgenerator_amd import manifest
 
 from .parity import assert_bitwise, assert_golden, run_case
 

@@ -6,7 +6,6 @@ Jacobi (the hip/benchmark.cpp:123-125 remark), linearity, the geometry's
 worked examples and invariants, and TwoSum's exactness for the FP32
 compensated variant."""
 import itertools
-import math
 from fractions import Fraction
 
 import numpy as np

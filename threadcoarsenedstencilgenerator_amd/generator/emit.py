@@ -44,7 +44,7 @@ from __future__ import annotations
 from collections import defaultdict
 
 from .. import spec
-from ..spec import (ALIGN_NONE, ALIGN_WRITE, BOX, COLUMN, F32C, F64, JACOBI, PLAIN, ROW, STAR,
+from ..spec import (ALIGN_NONE, ALIGN_WRITE, BOX, F32C, F64, PLAIN, ROW,
                     STREAM, VARIABLE, StencilDesc)
 
 # Byte offset given to lanes outside the interior.  The runtime only launches a

@@ -118,3 +118,20 @@ def test_geometry_check_via_c_abi():
     # grids smaller than the halo are rejected
     with pytest.raises(rt.TcsgError):
         rt.geometry(d, (8, 520, 520))
+
+
+def test_compiled_in_launch_unit_compiles(tmp_path):
+    """The generated kernels.cu (with the reference kernel file's symbols) and
+    the launch unit generateLaunchUpdateGrid.py prints compile together, with
+    both launch paths of the check harness, for a 2D Read-aligned Variable
+    kernel (host and gfx950 syntax)."""
+    from threadcoarsenedstencilgenerator_amd.build import HIP_FLAGS, HIPCC, INCLUDE
+    k = _gen("generate2DStencil.py", "Box", "Column", "Variable", "2", "2", "2", "Read", "8")
+    u = _gen("generateLaunchUpdateGrid.py", "Box", "Variable", "2", "2D", "Read")
+    assert k.returncode == 0 and u.returncode == 0
+    (tmp_path / "kernels.cu").write_text(k.stdout)
+    (tmp_path / "launchUpdateGrid.cu").write_text(u.stdout)
+    r = subprocess.run([HIPCC, *HIP_FLAGS, "-Wall", "-fsyntax-only", f"-I{INCLUDE}", f"-I{tmp_path}",
+                        os.path.join(PKG, "harness", "compiled_in_check.cpp")], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr[-3000:]

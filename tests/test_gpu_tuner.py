@@ -148,3 +148,21 @@ def test_tune_with_the_committed_model_on_unseen_kernels(tuner):
         k.close()
     print(f"regrets {[round(r, 4) for r in regrets]}", flush=True)
     assert np.mean(regrets) <= 0.10
+
+
+def test_tune_command_line():
+    """`tuner tune` on a kernel of the default build set: the model's top 8
+    shapes are timed and the fastest is reported as one JSON line."""
+    import json
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    argv = [sys.executable, "-m", "threadcoarsenedstencilgenerator_amd.tuner", "tune", "3D",
+            "Star Row Variable 4 1 4 16 Write 16 --max-threads=256", "--grid", "136", "72", "72"]
+    print(" ".join(argv[3:]), flush=True)
+    r = subprocess.run(argv, capture_output=True, text=True, timeout=240, cwd=root)
+    assert r.returncode == 0, r.stderr[-2000:]
+    out = json.loads([ln for ln in r.stdout.splitlines() if ln.startswith("{")][-1])
+    assert out["model"] and len(out["tried"]) == 8 and out["glups"] > 0
+    assert out["block"] in [b for b, _ in out["tried"]]

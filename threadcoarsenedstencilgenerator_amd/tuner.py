@@ -679,6 +679,12 @@ def main(argv=None):
     e.add_argument("--train-configs", type=int, default=0,
                    help="train on the first N training configurations only (0: all)")
     e.add_argument("--seed", type=int, default=0)
+    t = sub.add_parser("tune", help="block shape for one kernel: time the model's top k shapes")
+    t.add_argument("dim", choices=("2D", "3D"))
+    t.add_argument("kernel", help="the generator's argv in one string, e.g. "
+                                  "'Star Row Variable 4 2 4 128 Write 16 --xext=dppw'")
+    t.add_argument("--grid", type=int, nargs="+", required=True, help="grid size incl. halo, x y [z]")
+    t.add_argument("--k", type=int, default=8)
     f = sub.add_parser("fit")
     f.add_argument("--train", default=TRAIN_DATA)
     f.add_argument("--model", default=DEFAULT_MODEL)
@@ -727,6 +733,26 @@ def main(argv=None):
         if a.json:
             with open(a.json, "w") as fh:
                 json.dump(rep, fh, indent=1)
+        return 0
+    if a.cmd == "tune":
+        import torch
+
+        from . import build as B
+        from . import runtime as rt
+        from .generator.cli import desc_from_argv
+        torch.cuda.set_device(0)
+        dim = int(a.dim[0])
+        if len(a.grid) != dim:
+            raise SystemExit(f"--grid takes {dim} sizes for a {a.dim} kernel")
+        d, _ = desc_from_argv(dim, a.kernel.split())
+        path = B.build_kernel(d, force=False)   # compiled here unless the tree already holds it
+        k = rt.Kernel(d, path)
+        if d.coeff == VARIABLE:
+            k.set_mask([1.0 / d.n] * d.n)
+        block, glups, tried = tune(k, tuple(a.grid), k=a.k)
+        log(json.dumps({"kernel": d.key(), "grid": a.grid, "block": list(block), "glups": round(glups, 2),
+                        "model": os.path.exists(DEFAULT_MODEL),
+                        "tried": [[list(b), round(v, 2)] for b, v in tried]}))
         return 0
     if a.cmd == "fit":
         import torch

@@ -65,6 +65,13 @@ def _lib():
     if _L is None:
         if not os.path.exists(LIB):
             raise ImportError(f"{LIB} is not built; run __graft_entry__.build()")
+        # torch brings its own HIP runtime library; loaded first, it is the one libtcsg binds
+        # to as well.  Loaded after libtcsg (which would bring the system's), torch finds no
+        # GPU.  One runtime serves both, so device pointers and streams pass between them.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB)
         p, u32, i, u64 = C.c_void_p, C.c_uint32, C.c_int, C.c_uint64
         L.tcsg_geometry_compute.argtypes = [C.POINTER(CDesc), Dim3, C.POINTER(CGeometry)]

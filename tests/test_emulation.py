@@ -98,11 +98,11 @@ def _remainder_cases():
     for d, size, blocks in CASES:
         i = len(out) % 5
         out.append((d.replace(remainder=True), sizes[i], blocks))
-    # four FP32 columns per thread (one 16-byte store per output row) and three
+    # four FP64 columns per thread (two 16-byte stores per output row) and three
     # FP64 columns (a 16-byte store and a single one)
     base = dict(variant=spec.STREAM, xcd_remap=True, nt_store=True, remainder=True)
     out.append((StencilDesc(dimension=3, neighbourhood=STAR, orientation=ROW, coeff=VARIABLE, range=2, xc=4, yc=1,
-                            zc=2, alignment=ALIGN_WRITE, cl_size=16, dtype=F32C, comp_group=4, xext="dppw", **base),
+                            zc=2, alignment=ALIGN_WRITE, cl_size=16, xext="dppw", **base),
                 (27, 9, 10), [(64, 1, 1), (128, 1, 1)]))
     out.append((StencilDesc(dimension=2, neighbourhood=BOX, orientation=COLUMN, coeff=JACOBI, range=1, xc=3, yc=2,
                             alignment=ALIGN_NONE, cl_size=0, xext="dpp", **base), (33, 16), [(16, 4, 1), (96, 2, 1)]))
@@ -182,3 +182,13 @@ def test_remainder_geometry_limits():
     assert spec.geometry(p, (2 * r + 3, 2 * r + 3, 2 * r + 5)).threads == (2, 2, 2)
     with pytest.raises(ValueError):
         spec.geometry(p, (2 * r + 1, 2 * r + 2, 2 * r + 4))
+
+
+def test_fp32_stream_kernels_limited_to_two_columns_per_lane():
+    """FP32-storage STREAM kernels with more than two points per lane in x are
+    refused (measured wrong values on MI355X); PLAIN FP32 kernels are not limited."""
+    with pytest.raises(ValueError):
+        StencilDesc(dimension=3, neighbourhood=STAR, orientation=ROW, coeff=VARIABLE, range=4, xc=4, yc=2, zc=64,
+                    alignment=ALIGN_WRITE, cl_size=32, dtype=F32C, variant=spec.STREAM, xext="dppw")
+    StencilDesc(dimension=2, neighbourhood=BOX, orientation=ROW, coeff=VARIABLE, range=8, xc=4, yc=4,
+                alignment=ALIGN_READ, cl_size=16, dtype=F32C, variant=spec.PLAIN)

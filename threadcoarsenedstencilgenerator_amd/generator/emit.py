@@ -551,13 +551,19 @@ class _Emitter:
                 w("  const u32 tL = tcsg_min(tx, lane);")
                 w("  const u32 tR = tcsg_min(blockDim.x - 1u - tx, 63u - lane);")
                 w(f"  const lbool edgeL = tL < {H}u, edgeR = tR < {H}u;")
+        # the left edge lanes' extra columns start H*XC columns left of x0; x0 >= R + LO,
+        # so with R + LO >= H*XC one offset (plus immediates) serves all of them and the
+        # compiler merges their loads into one wide load per row
+        merged_left = R + LO >= H * XC
         if dpp:
-            w("  // Extra columns of the segment-edge lanes.  Left ones are addressed from their")
-            w("  // own offsets, only where the column index stays non-negative, so no per-lane")
-            w("  // offset wraps below zero.")
-            for ip in range(XC):
-                k = H * XC - ip
-                w(f"  const u32 eoffL{ip} = tcsg_sel(edgeL && x0 >= {k}u, base - {k * es}u, {OOB_BASE:#x}u);")
+            w("  // Extra columns of the segment-edge lanes.  Left ones are addressed only where")
+            w("  // the column index stays non-negative, so no per-lane offset wraps below zero.")
+            if merged_left:
+                w(f"  const u32 eoffL = tcsg_sel(edgeL, base - {H * XC * es}u, {OOB_BASE:#x}u);")
+            else:
+                for ip in range(XC):
+                    k = H * XC - ip
+                    w(f"  const u32 eoffL{ip} = tcsg_sel(edgeL && x0 >= {k}u, base - {k * es}u, {OOB_BASE:#x}u);")
             w(f"  const u32 eoffR = tcsg_sel(edgeR, base + {H * XC * es}u, {OOB_BASE:#x}u);")
         pts, contrib = self.footprint()
         last = {}
@@ -612,7 +618,10 @@ class _Emitter:
                     if sp[0] in ("EL", "ER"):
                         _, ip, py2, pz2 = sp
                         side = sp[0][1]
-                        off = f"eoffL{ip}" if side == "L" else f"eoffR + {ip * es}u"
+                        if side == "R":
+                            off = f"eoffR + {ip * es}u"
+                        else:
+                            off = f"eoffL + {ip * es}u" if merged_left else f"eoffL{ip}"
                         out.append((f"e{side}0_{ip}_{_n(py2)}_{_n(pz2)}", off, soff(py2, pz2)))
                     else:
                         qx, qy, qz = sp

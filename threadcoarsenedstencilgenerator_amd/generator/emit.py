@@ -565,6 +565,12 @@ class _Emitter:
                     k = H * XC - ip
                     w(f"  const u32 eoffL{ip} = tcsg_sel(edgeL && x0 >= {k}u, base - {k * es}u, {OOB_BASE:#x}u);")
             w(f"  const u32 eoffR = tcsg_sel(edgeR, base + {H * XC * es}u, {OOB_BASE:#x}u);")
+        # wave-aligned rows: the left edge lanes (lane < H) and the right ones (lane > 63 - H)
+        # never coincide, so one load, with each side's offset in its own lanes, fetches both
+        # sides' extra columns into one register that both lane-shift chains read
+        both_edges = dpp and wave_aligned and merged_left
+        if both_edges:
+            w("  const u32 eoffE = tcsg_sel(edgeL, eoffL, eoffR);")
         pts, contrib = self.footprint()
         last = {}
         for idx, p in enumerate(pts):
@@ -577,6 +583,12 @@ class _Emitter:
 
         def vname(px, py, pz):
             return f"v_{_n(px)}_{_n(py)}_{_n(pz)}"
+
+        def ename(side, j, ip, py, pz):
+            """Edge-column value j lane hops along the chain of `side` (j = 0: as loaded)."""
+            if j == 0 and both_edges:
+                return f"eE0_{ip}_{_n(py)}_{_n(pz)}"
+            return f"e{side}{j}_{ip}_{_n(py)}_{_n(pz)}"
 
         def hop(px):
             """(side, h, ip): lane hop and source column of an x-neighbour."""
@@ -618,11 +630,13 @@ class _Emitter:
                     if sp[0] in ("EL", "ER"):
                         _, ip, py2, pz2 = sp
                         side = sp[0][1]
-                        if side == "R":
+                        if both_edges:
+                            off = f"eoffE + {ip * es}u"
+                        elif side == "R":
                             off = f"eoffR + {ip * es}u"
                         else:
                             off = f"eoffL + {ip * es}u" if merged_left else f"eoffL{ip}"
-                        out.append((f"e{side}0_{ip}_{_n(py2)}_{_n(pz2)}", off, soff(py2, pz2)))
+                        out.append((ename(side, 0, ip, py2, pz2), off, soff(py2, pz2)))
                     else:
                         qx, qy, qz = sp
                         out.append((vname(qx, qy, qz), f"base + {(qx + XL) * es}u", soff(qy, qz)))
@@ -654,8 +668,7 @@ class _Emitter:
                         continue
                     side, h, ip = hop(px)
                     main_fn, e_fn = ("tcsg_shr1", "tcsg_shl1") if side == "L" else ("tcsg_shl1", "tcsg_shr1")
-                    en = f"e{side}0_{ip}_{_n(py)}_{_n(pz)}"
-                    gname = lambda j: f"e{side}{j}_{ip}_{_n(py)}_{_n(pz)}"
+                    gname = lambda j: ename(side, j, ip, py, pz)
                     for j in range(1, H):
                         if gname(j) not in declared:
                             declared.add(gname(j))

@@ -568,7 +568,7 @@ class _Emitter:
         # wave-aligned rows: the left edge lanes (lane < H) and the right ones (lane > 63 - H)
         # never coincide, so one load, with each side's offset in its own lanes, fetches both
         # sides' extra columns into one register that both lane-shift chains read
-        both_edges = dpp and wave_aligned and merged_left
+        both_edges = dpp and wave_aligned and merged_left and 2 * H <= 64
         if both_edges:
             w("  const u32 eoffE = tcsg_sel(edgeL, eoffL, eoffR);")
         pts, contrib = self.footprint()

@@ -7,7 +7,10 @@ One call of :func:`emit_source` produces a translation unit with
   hip/generate3DStencil_hip.py:242-384), taking a ``tcsg_kargs`` struct;
 * ``tcsg_params`` -- the compile-time parameters, which the runtime compares
   with the caller's descriptor before any launch (the reference's compiled-in
-  sanity check, hip/generateLaunchUpdateGrid.py:106-119).
+  sanity check, hip/generateLaunchUpdateGrid.py:106-119);
+* optionally ``tcsg_update_batched`` (the persistent kernel over a tile
+  table) and the reference kernel file's ``updateGrid``, ``init``, ``d_mask``
+  and ``range``.
 
 Arithmetic contract, shared by every variant:  each output point sums its
 neighbourhood in the reference's traversal order (Row (dz,dy,dx), Column
@@ -35,7 +38,9 @@ STREAM  the MI355X shape of the same computation:
           segment) needs r columns beyond each of its ends; the H = ceil(r/xc)
           lanes at each end load those with one extra load per row, and a
           second DPP chain hands them to the lanes that need them, so no lane
-          loads an x-neighbour individually;
+          loads an x-neighbour individually.  ``xext="dppw"`` (block rows of
+          whole waves) fetches both ends' columns with one load per row, each
+          end's lanes using their own offset;
         * optional XCD-aware block remapping (consecutive tiles on one XCD's
           L2) and nontemporal stores.
 """

@@ -65,3 +65,52 @@ def test_batched_matches_oracle(dtype, mode, shape, block, workgroups):
         ui = np.uint32 if dtype == F32C else np.uint64
         assert np.array_equal(got.view(ui), ref.view(ui)), f"grid {size} differs"
     b.close()
+
+
+@pytest.mark.parametrize("dtype", [F64, F32C])
+def test_graph_replays_with_work_between(dtype):
+    """The captured step loop replayed four times from the same initial state,
+    with other GPU work (the plain kernel on another grid, a torch reduction)
+    and host synchronisation between replays: every replay equals the oracle,
+    so each launch found the work queue cleared by the previous one."""
+    import torch
+
+    from threadcoarsenedstencilgenerator_amd import runtime as rt
+    d = manifest.cfg4_desc(shape="plain", batched=True, dtype=dtype, max_threads=1024)
+    k = rt.Kernel(d)
+    sizes = rt.batch_sizes(5, 7, 8, 40, d.coarsening, d.range)
+    print(f"graph replays {d.key()} sizes {sizes}", flush=True)
+    geoms, grids = [], []
+    for i, size in enumerate(sizes):
+        g = k.geometry(size)
+        gr = rt.Grids(g, k.dtype)
+        for j in range(2):
+            k.init(g, gr.ptr(j), rt.INIT_U01, 40 + i)
+        geoms.append(g)
+        grids.append(gr)
+    torch.cuda.synchronize()
+    init = [gr.host(0) for gr in grids]
+    steps = 3
+    run = oracle.run_f32c if dtype == F32C else oracle.run_f64
+    refs = [run(d, oracle.Geometry(3, d.range, d.alignment, d.cl_size, size, d.coarsening), None, a.copy(),
+                a.copy(), steps) for size, a in zip(sizes, init)]
+    other = manifest.cfg4_desc(shape="plain", dtype=dtype)
+    ko = rt.Kernel(other)
+    go = ko.geometry((28, 20, 24))
+    gro = rt.Grids(go, ko.dtype)
+    b = rt.Batch(k, geoms, [gr.ptr(0) for gr in grids], [gr.ptr(1) for gr in grids], (32, 8, 2), workgroups=0)
+    ui = np.uint32 if dtype == F32C else np.uint64
+    for replay in range(4):
+        for gr, a in zip(grids, init):
+            gr.upload(0, a)
+            gr.upload(1, a)
+        torch.cuda.synchronize()
+        fin = b.run(steps, graph=True)
+        torch.cuda.synchronize()
+        for gr, ref, size in zip(grids, refs, sizes):
+            assert np.array_equal(gr.host(fin).view(ui), ref.view(ui)), f"replay {replay}: grid {size} differs"
+        # unrelated work between replays
+        ko.run(go, (32, 4, 2), rt.grid_blocks(go, (32, 4, 2)), gro.ptr(0), gro.ptr(1), 2)
+        float(gro.bufs[0].sum())
+        torch.cuda.synchronize()
+    b.close()

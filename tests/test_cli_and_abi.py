@@ -51,7 +51,11 @@ def test_generator_argv_like_reference():
     assert re.search(r"tcsg_params\[24\] = \{(\d+, ){18}5,", r.stdout)   # nt_store | remainder << 2
     r = _gen("generate2DStencil.py", "Box", "Row", "Variable", "8", "4", "4", "--variant=plain", "--order=offset")
     assert r.returncode == 0 and "order=offset" in r.stdout
-    for extra in (("--order=offset",),):
+    # paired edge-column loads: the DPP row shifts that move the second row's columns into place
+    r = _gen("generate3DStencil.py", "Star", "Row", "Variable", "4", "2", "4", "8", "Write", "16", "--xext=dppw",
+             "--edge-pairs")
+    assert r.returncode == 0 and "eoffP" in r.stdout and "tcsg_rowshl2(eP_" in r.stdout   # H = ceil(4 / 2)
+    for extra in (("--order=offset",), ("--edge-pairs",)):
         r = _gen("generate2DStencil.py", "Box", "Row", "Jacobi", "1", "2", "1", *extra)
         assert r.returncode == 1 and "usage" in r.stderr
 

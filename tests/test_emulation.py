@@ -45,7 +45,7 @@ def test_emulated_stream_kernel_matches_oracle(case):
     geo = spec.geometry(d, size)
     og = oracle.Geometry(d.dimension, d.range, d.alignment, d.cl_size, size, d.coarsening)
     a = np.zeros(og.n_points)
-    oracle.init_seeded(og, 1, 4, a)
+    oracle.init_seeded(og, 1, 4, a, halo=0.375)
     mask = oracle.mask_seeded(6, d.n)
     dt = np.float32 if d.dtype == F32C else np.float64
     a = a.astype(dt)
@@ -106,6 +106,9 @@ def _remainder_cases():
                 (27, 9, 10), [(64, 1, 1), (128, 1, 1)]))
     out.append((StencilDesc(dimension=2, neighbourhood=BOX, orientation=COLUMN, coeff=JACOBI, range=1, xc=3, yc=2,
                             alignment=ALIGN_NONE, cl_size=0, xext="dpp", **base), (33, 16), [(16, 4, 1), (96, 2, 1)]))
+    # wave-aligned kernels with paired edge loads over rows of more than two waves
+    out += [(d.replace(edge_pairs=True), (size[0] + 150 * d.xc + 1, *size[1:]), blocks)
+            for d, size, blocks in out[:len(CASES)] if d.xext == "dppw"]
     return out
 
 
@@ -125,7 +128,7 @@ def test_emulated_remainder_kernel_matches_oracle(case):
     assert (geo.ydim, geo.zdim) == (tuple(size) + (1,))[1:3]
     assert any(e % c for e, c in zip(geo.elements, d.coarsening))
     a = np.zeros(og.n_points)
-    oracle.init_seeded(og, 1, 4, a)
+    oracle.init_seeded(og, 1, 4, a, halo=0.375)
     mask = oracle.mask_seeded(6, d.n)
     dt = np.float32 if d.dtype == F32C else np.float64
     a = a.astype(dt)
@@ -192,3 +195,34 @@ def test_fp32_stream_kernels_limited_to_two_columns_per_lane():
                     alignment=ALIGN_WRITE, cl_size=32, dtype=F32C, variant=spec.STREAM, xext="dppw")
     StencilDesc(dimension=2, neighbourhood=BOX, orientation=ROW, coeff=VARIABLE, range=8, xc=4, yc=4,
                 alignment=ALIGN_READ, cl_size=16, dtype=F32C, variant=spec.PLAIN)
+
+
+# the wave-aligned cases with x widened past two waves of threads, so that waves also meet inside
+# the interior and their edge lanes carry the neighbouring wave's columns (the halo is nonzero, so
+# the grid's own border counts too); each also with edge_pairs
+WIDE_CASES = [(d.replace(edge_pairs=pairs), (size[0] + 150 * d.xc, *size[1:]), blocks)
+              for d, size, blocks in CASES if d.xext == "dppw" for pairs in (False, True)]
+
+
+@pytest.mark.parametrize("case", WIDE_CASES, ids=[c[0].key() for c in WIDE_CASES])
+def test_emulated_wide_rows_match_oracle(case):
+    """Wave-aligned kernels over rows of several waves equal the oracle, also
+    when the edge lanes fetch two rows' extra columns per load (the second
+    row's moved into place by a DPP row shift)."""
+    d, size, blocks = case
+    geo = spec.geometry(d, size)
+    og = oracle.Geometry(d.dimension, d.range, d.alignment, d.cl_size, size, d.coarsening)
+    a = np.zeros(og.n_points)
+    oracle.init_seeded(og, 1, 4, a, halo=0.375)
+    mask = oracle.mask_seeded(6, d.n)
+    dt = np.float32 if d.dtype == F32C else np.float64
+    a = a.astype(dt)
+    run = oracle.run_f32c if d.dtype == F32C else oracle.run_f64
+    ref = run(d, og, mask, a.copy(), a.copy(), 2)
+    for block in blocks:
+        nblk = tuple((geo.threads[i] + block[i] - 1) // block[i] for i in range(3))
+        got, bad = emulate.run(d, geo, mask.astype(dt), a.copy(), a.copy(), 2, block, nblk)
+        assert bad == 0
+        ui = np.uint32 if dt == np.float32 else np.uint64
+        diff = np.flatnonzero(got.view(ui) != ref.view(ui))
+        assert diff.size == 0, f"block {block}: {diff.size} points differ, first {diff[:5]}"

@@ -13,7 +13,8 @@ shape (no reference counterpart) are given as --name=value:
 --max-threads=N, --prefetch=N, --waves-per-eu=N, --comp-group=N,
 --order=point|offset (plain variant), --no-xcd-remap, --no-nt-store, --zfirst,
 --remainder (grids whose interior is not a multiple of the coarsening are
-updated whole instead of truncated).
+updated whole instead of truncated), --edge-pairs (--xext=dppw: the wave-edge
+lanes fetch two adjacent rows' extra columns per load).
 Invalid arguments print the usage on stderr and exit with status 1, as the
 reference does.
 """
@@ -99,7 +100,7 @@ def desc_from_argv(dim: int, argv):
               variant=spec.VARIANT_NAMES[opts.get("variant", "stream")],
               dtype=spec.DTYPE_NAMES[opts.get("dtype", "f64")],
               xext=opts.get("xext", "dpp"), xcd_remap="no-xcd-remap" not in opts, nt_store="no-nt-store" not in opts,
-              zfirst="zfirst" in opts, remainder="remainder" in opts)
+              zfirst="zfirst" in opts, remainder="remainder" in opts, edge_pairs="edge-pairs" in opts)
     for k, f in (("max-threads", "max_threads"), ("prefetch", "prefetch"), ("waves-per-eu", "waves_per_eu"),
                  ("comp-group", "comp_group")):
         if k in opts:

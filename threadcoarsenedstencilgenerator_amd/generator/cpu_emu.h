@@ -151,6 +151,25 @@ Lane<T> tcsg_shl1_or(const Lane<T> &old, const Lane<T> &v)
     return r;
 }
 
+// DPP row_shl:n (lane i <- lane i+n) and row_shr:n (lane i <- lane i-n), each within its
+// row of 16 lanes; lanes whose source lies outside the row get 0.
+template <int N, class T>
+Lane<T> tcsg_emu_rowshl(const Lane<T> &v)
+{
+    Lane<T> r(0);
+    for (int i = 0; i < tcsg_emu::W; ++i)
+        if ((i & 15) + N < 16) r.v[i] = v.v[i + N];
+    return r;
+}
+template <int N, class T>
+Lane<T> tcsg_emu_rowshr(const Lane<T> &v)
+{
+    Lane<T> r(0);
+    for (int i = 0; i < tcsg_emu::W; ++i)
+        if ((i & 15) >= N) r.v[i] = v.v[i - N];
+    return r;
+}
+
 struct tcsg_emu_rsrc {
     char *base;
     uint32_t records;

@@ -61,6 +61,8 @@ OOB_BASE = 0xC0000000
 
 DPP_WAVE_SHL1 = 0x130  # lane i receives lane i+1
 DPP_WAVE_SHR1 = 0x138  # lane i receives lane i-1
+DPP_ROW_SHL = 0x100     # + n: lane i receives lane i+n within its row of 16 lanes
+DPP_ROW_SHR = 0x110     # + n: lane i receives lane i-n within its row of 16 lanes
 
 
 def _n(v: int) -> str:
@@ -234,6 +236,22 @@ class _Emitter:
                     w("  const int olo = __double2loint(old), ohi = __double2hiint(old);")
                     w(f"  return __hiloint2double(__builtin_amdgcn_update_dpp(ohi, hi, {ctrl:#x}, 0xf, 0xf, false),")
                     w(f"                          __builtin_amdgcn_update_dpp(olo, lo, {ctrl:#x}, 0xf, 0xf, false));")
+                    w("}")
+        if d.edge_pairs:
+            H = self.halo_lanes()
+            for nm, ctrl in (("rowshl", DPP_ROW_SHL + H), ("rowshr", DPP_ROW_SHR + H)):
+                if cpu:
+                    w(f"inline TV tcsg_{nm}{H}(const TV& v) {{ return tcsg_emu_{nm}<{H}>(v); }}")
+                elif self.f32:
+                    w(f"__device__ __forceinline__ float tcsg_{nm}{H}(float v) {{")
+                    w(f"  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp("
+                      f"__builtin_bit_cast(int, v), {ctrl:#x}, 0xf, 0xf, true));")
+                    w("}")
+                else:
+                    w(f"__device__ __forceinline__ double tcsg_{nm}{H}(double v) {{")
+                    w("  const int lo = __double2loint(v), hi = __double2hiint(v);")
+                    w(f"  return __hiloint2double(__builtin_amdgcn_mov_dpp(hi, {ctrl:#x}, 0xf, 0xf, true),")
+                    w(f"                          __builtin_amdgcn_mov_dpp(lo, {ctrl:#x}, 0xf, 0xf, true));")
                     w("}")
         if d.variant == STREAM:
             w("// Buffer access: per-lane byte offset `off` (range-checked: lanes given an")
@@ -577,6 +595,29 @@ class _Emitter:
         if both_edges:
             w("  const u32 eoffE = tcsg_sel(edgeL, eoffL, eoffR);")
         pts, contrib = self.footprint()
+        # edge_pairs: two adjacent rows of one plane that both need edge columns share a load;
+        # the second row's columns go to the next H lanes on each side (one row below in the
+        # per-lane offset) and a DPP row shift moves them into the edge lanes
+        pair_role = {}
+        pair_edges = d.edge_pairs and both_edges and 2 * H <= 16
+        if pair_edges:
+            need = sorted({(pz, py) for (px, py, pz) in pts if not 0 <= px < XC})
+            rows = defaultdict(list)
+            for pz, py in need:
+                rows[pz].append(py)
+            for pz, pys in rows.items():
+                i = 0
+                while i + 1 < len(pys):
+                    if pys[i + 1] == pys[i] + 1:
+                        pair_role[(pys[i], pz)] = ("a", pys[i])
+                        pair_role[(pys[i + 1], pz)] = ("b", pys[i])
+                        i += 2
+                    else:
+                        i += 1
+            w(f"  const lbool pairL = lane >= {H}u && lane < {2 * H}u;")
+            w(f"  const lbool pairR = lane + {2 * H}u > 63u && lane + {H}u <= 63u;")
+            w(f"  const u32 eoffP = tcsg_sel(pairL, base - {2 * H * XC * es}u + sx, "
+              f"tcsg_sel(pairR, base + {2 * H * XC * es}u + sx, eoffE));")
         last = {}
         for idx, p in enumerate(pts):
             for (o, t, m) in contrib[p]:
@@ -590,7 +631,11 @@ class _Emitter:
             return f"v_{_n(px)}_{_n(py)}_{_n(pz)}"
 
         def ename(side, j, ip, py, pz):
-            """Edge-column value j lane hops along the chain of `side` (j = 0: as loaded)."""
+            """Edge-column value j lane hops along the chain of `side` (j = 0: as loaded,
+            or for the second row of a pair, as moved into the edge lanes)."""
+            if j == 0 and (py, pz) in pair_role:
+                role, pya = pair_role[(py, pz)]
+                return f"eP_{ip}_{_n(pya)}_{_n(pz)}" if role == "a" else f"eP{side}_{ip}_{_n(pya)}_{_n(pz)}"
             if j == 0 and both_edges:
                 return f"eE0_{ip}_{_n(py)}_{_n(pz)}"
             return f"e{side}{j}_{ip}_{_n(py)}_{_n(pz)}"
@@ -635,6 +680,10 @@ class _Emitter:
                     if sp[0] in ("EL", "ER"):
                         _, ip, py2, pz2 = sp
                         side = sp[0][1]
+                        if (py2, pz2) in pair_role:
+                            pya = pair_role[(py2, pz2)][1]
+                            out.append((f"eP_{ip}_{_n(pya)}_{_n(pz2)}", f"eoffP + {ip * es}u", soff(pya, pz2)))
+                            continue
                         if both_edges:
                             off = f"eoffE + {ip * es}u"
                         elif side == "R":
@@ -674,6 +723,11 @@ class _Emitter:
                     side, h, ip = hop(px)
                     main_fn, e_fn = ("tcsg_shr1", "tcsg_shl1") if side == "L" else ("tcsg_shl1", "tcsg_shr1")
                     gname = lambda j: ename(side, j, ip, py, pz)
+                    if (py, pz) in pair_role and pair_role[(py, pz)][0] == "b" and gname(0) not in declared:
+                        pya = pair_role[(py, pz)][1]
+                        declared.add(gname(0))
+                        w(f"  const TV {gname(0)} = tcsg_row{'shl' if side == 'L' else 'shr'}{H}("
+                          f"eP_{ip}_{_n(pya)}_{_n(pz)});")
                     for j in range(1, H):
                         if gname(j) not in declared:
                             declared.add(gname(j))
@@ -788,7 +842,7 @@ class _Emitter:
         if fallback:
             device_body("tcsg_step_plain", d.replace(variant=PLAIN, xext="load", xcd_remap=False, nt_store=False,
                                                      nt_load=False, zfirst=False, prefetch=0, waves_per_eu=0,
-                                                     batched=False, order="point"))
+                                                     batched=False, order="point", edge_pairs=False))
         args = "const unsigned long xdim, const unsigned long ydim"
         if three:
             args += ", const unsigned long zdim, double * const __restrict__ array0"

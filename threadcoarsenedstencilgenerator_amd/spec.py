@@ -71,6 +71,8 @@ class StencilDesc:
     #                          end at the interior's end (STREAM kernels in x instead drop the
     #                          columns past the interior); off, axes are truncated as the
     #                          reference does
+    edge_pairs: bool = False  # dppw: the edge columns of two adjacent rows in one load (the
+    #                          second row's in the next edge lanes, moved by a DPP row shift)
     order: str = "point"     # PLAIN statement order: "point" loads each footprint point once and
     #                          adds it to every output that uses it; "offset" walks the mask in
     #                          summation order, every output per offset (weights used in order,
@@ -112,6 +114,8 @@ class StencilDesc:
             # returned a varying number of wrong points from run to run, where xc <= 2 and
             # every FP64 kernel are bit-exact; the CPU emulation of the same text is exact
             raise ValueError("FP32-storage STREAM kernels take at most 2 points per lane in x")
+        if self.edge_pairs and (self.variant != STREAM or self.xext != "dppw"):
+            raise ValueError("edge_pairs is an option of the wave-aligned DPP code shape")
 
     # ---- derived quantities ----
     @property
@@ -170,7 +174,7 @@ class StencilDesc:
 
 # knobs added after kernel names were first recorded (measured corpora refer to
 # kernels by name): left out of the name's hash while at their default
-_LATER_FIELDS = {"nt_load": False, "order": "point", "remainder": False}
+_LATER_FIELDS = {"nt_load": False, "order": "point", "remainder": False, "edge_pairs": False}
 
 
 def mask_index(desc: StencilDesc, dx: int, dy: int, dz: int = 0):

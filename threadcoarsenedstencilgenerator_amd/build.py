@@ -93,6 +93,20 @@ def build_kernels(descs, force: bool = True, jobs: int | None = None):
     return out
 
 
+def prune_kernels(descs) -> int:
+    """Delete code objects (and their .sha1 sidecars) in the kernel directory
+    that belong to none of `descs`, the range-check probe kept; returns how
+    many code objects went."""
+    keep = {d.key() for d in descs} | {os.path.splitext(os.path.basename(PROBE_CO))[0]}
+    gone = 0
+    for f in os.listdir(KDIR):
+        stem, ext = os.path.splitext(f)
+        if ext in (".co", ".sha1") and stem not in keep:
+            os.remove(os.path.join(KDIR, f))
+            gone += ext == ".co"
+    return gone
+
+
 # Kernels built the reference's way: kernels.cu (the kernel generator's output, with the
 # reference kernel file's symbols) compiled into the launch unit generateLaunchUpdateGrid.py
 # prints, under harness/compiled_in/<name>/, with a check harness of our own that drives both

@@ -22,9 +22,11 @@ CFG2_F64 = dict(desc=manifest.cfg2_desc(2, 4, 128, shape="dppw", max_threads=256
 # FP32 rows are aligned to 32 points = 128 bytes, the line size the FP64 grid's 16 points
 # make (16 FP32 points, 64 bytes, leaves each row end's line written by two tiles: 5.6%
 # more write traffic); the first group sums the 21 terms up to the output's own plane,
-# the second the 4 terms of the planes after it, and TwoSum joins them.
+# the second the 4 terms of the planes after it, and TwoSum joins them.  The edge lanes fetch
+# the extra columns of two adjacent rows per load (edge_pairs: 16% fewer buffer loads, 3.7%
+# faster; the FP64 kernel gains nothing from it).
 CFG2_F32C = dict(desc=manifest.cfg2_desc(2, 8, 64, shape="dppw", max_threads=256, dtype=F32C).replace(
-    prefetch=0, comp_group=21, zfirst=False, cl_size=32), block=(64, 4, 1))
+    prefetch=0, comp_group=21, zfirst=False, cl_size=32, edge_pairs=True), block=(128, 2, 1))
 
 
 # 2D Box Variable r=8 (289 weights), 16384x8192 interior, Read alignment at 16 points

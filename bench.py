@@ -14,6 +14,7 @@ write, against 8 TB/s), and the OpenMP CPU oracle timed on this host.
     python bench.py [--gpus 1] [--steps 100] [--warmup 5]
     python bench.py --workload batched     # 64 uneven 3D Box grids, persistent kernel + hipGraph
     python bench.py --workload box2d_r8    # 2D Box r=8 16384x8192, FP64 and FP32-compensated
+    python bench.py --workload box2d_r1_temporal   # 2D Box r=1 16384^2, several steps per launch
 """
 from __future__ import annotations
 
@@ -290,6 +291,65 @@ def bench_remainder(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_temporal(args):
+    """2D Box Jacobi r=1 (the 1024^2 workload's stencil) on a 16384x16384
+    interior, FP64, U[0,1) values: the temporal kernel (several steps per
+    launch through LDS) against the fastest one-step kernel on the same grid.
+    Both run the same number of time steps; the one-step kernel's roofline is
+    its 16 B per update of HBM traffic, which the temporal kernel spends once
+    per launch instead of once per step."""
+    import torch
+
+    from threadcoarsenedstencilgenerator_amd import runtime as rt
+    from threadcoarsenedstencilgenerator_amd import tuned
+    tcfg, ocfg = tuned.BOX2D_R1_TEMPORAL, tuned.BOX2D_R1_ONE_STEP
+    stream = torch.cuda.current_stream()
+    out = {}
+    for name, cfg in (("temporal", tcfg), ("one_step", ocfg)):
+        k = rt.Kernel(cfg["desc"])
+        block = tuple(cfg["block"])
+        K = k.steps_per_launch
+        launches = max(1, args.steps // K)
+        g = k.geometry(tuned.BOX2D_R1_GRID)
+        grids = rt.Grids(g, k.dtype)
+        for i in range(2):
+            k.init(g, grids.ptr(i), rt.INIT_U01, INPUT_SEED)
+        blocks = rt.grid_blocks(g, block)
+        k.run(g, block, blocks, grids.ptr(0), grids.ptr(1), max(1, args.warmup // K), stream)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        k.run(g, block, blocks, grids.ptr(0), grids.ptr(1), launches, stream)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        ms = k.time_steps(g, block, blocks, grids.ptr(0), grids.ptr(1), 0, launches, stream) / launches
+        pts = g.interior_points
+        out[name] = {"glups": round(pts * K * launches / dt / 1e9, 2), "steps": K * launches,
+                     "steps_per_launch": K, "kernel_ms_per_launch": round(ms, 4),
+                     "kernel_glups": round(pts * K / (ms * 1e-3) / 1e9, 2), "block": list(block),
+                     "kernel": cfg["desc"].key()}
+        del grids
+        k.close()
+        torch.cuda.empty_cache()
+    t, o = out["temporal"], out["one_step"]
+    pts = (tuned.BOX2D_R1_GRID[0] - 2) * (tuned.BOX2D_R1_GRID[1] - 2)
+    alg = 16.0 * pts / (t["kernel_ms_per_launch"] * 1e-3) / 1e9   # one read + one write per point per launch
+    line = {"metric": "GLup/s, FP64 2D Box Jacobi r=1 16384^2, temporal blocking", "value": t["glups"],
+            "unit": "GLup/s", "n_gpus": 1, "steps": t["steps"], "warmup": args.warmup, "higher_is_better": True,
+            "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "2D Box Jacobi r=1, 16384x16384 interior + 1-point halo, FP64, alignment None, "
+                                   "U[0,1) values",
+                       "steps_per_launch": t["steps_per_launch"], "tile": list(tcfg["desc"].tile[:2]),
+                       "block": t["block"]},
+            "roofline": {"bound": "hbm", "achieved": round(alg, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(alg / HBM_PEAK_GBS, 4), "traffic": None,
+                         "one_step_equivalent_gbs": round(16.0 * t["kernel_glups"], 1),
+                         "note": "achieved: 16 B per point per launch (each launch advances steps_per_launch "
+                                 "steps); one_step_equivalent_gbs: the HBM rate a one-step kernel would need "
+                                 "for the same updates per second"},
+            "vs_one_step": round(t["glups"] / o["glups"], 3), "runs": out}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -298,7 +358,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--workload", default="star3d_r4_512", choices=("star3d_r4_512", "batched", "box2d_r8",
-                                                                    "star3d_r4_511_remainder"))
+                                                                    "star3d_r4_511_remainder", "box2d_r1_temporal"))
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
@@ -318,6 +378,8 @@ def main():
         return bench_box2d_r8(args)
     if args.workload == "star3d_r4_511_remainder":
         return bench_remainder(args)
+    if args.workload == "box2d_r1_temporal":
+        return bench_temporal(args)
     best = tuned.best("star3d_r4_512")
     desc, block = best["desc"], tuple(best["block"])
     k = rt.Kernel(desc)

@@ -32,7 +32,7 @@ enum { TCSG_ROW = 0, TCSG_COLUMN = 1 };
 enum { TCSG_JACOBI = 0, TCSG_VARIABLE = 1 };
 enum { TCSG_ALIGN_NONE = 0, TCSG_ALIGN_READ = 1, TCSG_ALIGN_WRITE = 2 };
 enum { TCSG_F64 = 0, TCSG_F32C = 1 };
-enum { TCSG_PLAIN = 0, TCSG_STREAM = 1 };
+enum { TCSG_PLAIN = 0, TCSG_STREAM = 1, TCSG_TEMPORAL = 2 };
 
 /* initial-value modes for tcsg_launch_init */
 enum {
@@ -69,7 +69,7 @@ typedef struct tcsg_desc {
     uint32_t alignment;      /* TCSG_ALIGN_*                */
     uint32_t cl_size;        /* alignment size in points    */
     uint32_t dtype;          /* TCSG_F64 / TCSG_F32C        */
-    uint32_t variant;        /* TCSG_PLAIN / TCSG_STREAM    */
+    uint32_t variant;        /* TCSG_PLAIN / TCSG_STREAM / TCSG_TEMPORAL */
 } tcsg_desc;
 
 /* Padded geometry of a grid (hip/benchmark.cpp:140-177, hip/benchmark.h:100-123). */
@@ -117,6 +117,12 @@ int tcsg_kernel_desc(const tcsg_kernel *k, tcsg_desc *out);
 int tcsg_kernel_geometry(const tcsg_kernel *k, tcsg_dim3 size, tcsg_geometry *out);
 /* *out = 1 for a kernel generated with remainder handling, else 0. */
 int tcsg_kernel_remainder(const tcsg_kernel *k, int *out);
+/* *out = time steps one launch advances: 1, or for a TCSG_TEMPORAL kernel (2D,
+ * no reference counterpart) the steps it runs per launch in LDS; tcsg_update_grid,
+ * tcsg_run and tcsg_time_steps then count launches, each of that many steps.
+ * A temporal kernel runs with block (64, launch bound / 64, 1) only and keeps
+ * every interior point whatever the remainder option. */
+int tcsg_kernel_steps_per_launch(const tcsg_kernel *k, int *out);
 
 /* Variable-mask upload (n doubles in the layout of the reference's d_mask).
  * Replaces copyMask (hip/launchUpdateGrid.h:18, hip/generateLaunchUpdateGrid.py:77-82).

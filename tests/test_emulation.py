@@ -3,7 +3,9 @@ the host through the wave-level CPU emulation, bit for bit against the
 oracle, with no buffer access that would pass the hardware's per-lane range
 check while lying past the buffer.  Covers the x-neighbour exchange modes
 (loads, segment DPP, wave-aligned DPP), both dtypes and several block
-shapes, including blocks whose rows span or split waves."""
+shapes, including blocks whose rows span or split waves.  The temporal
+kernels' phases (load, sub-steps, store; the barriers between them) run
+for every thread of a workgroup in turn."""
 import numpy as np
 import pytest
 
@@ -226,3 +228,46 @@ def test_emulated_wide_rows_match_oracle(case):
         ui = np.uint32 if dt == np.float32 else np.uint64
         diff = np.flatnonzero(got.view(ui) != ref.view(ui))
         assert diff.size == 0, f"block {block}: {diff.size} points differ, first {diff[:5]}"
+
+
+TEMPORAL_CASES = manifest.temporal_cases()
+
+
+@pytest.mark.parametrize("case", TEMPORAL_CASES, ids=[c[0].key() for c in TEMPORAL_CASES])
+def test_emulated_temporal_kernel_matches_oracle(case):
+    """Two launches of a kernel that advances tsteps steps per launch equal
+    2*tsteps steps of the oracle bit for bit, halo and padding included."""
+    d, size = case
+    geo = spec.geometry(d, size)
+    og = oracle.Geometry(2, d.range, d.alignment, d.cl_size, size, (1, 1, 1) if d.remainder else d.coarsening)
+    assert (og.xdim, og.ydim) == (geo.xdim, geo.ydim)
+    a = np.zeros(og.n_points)
+    oracle.init_seeded(og, 1, 4, a, halo=0.375)
+    mask = oracle.mask_seeded(6, d.n)
+    dt = np.float32 if d.dtype == F32C else np.float64
+    a = a.astype(dt)
+    run = oracle.run_f32c if d.dtype == F32C else oracle.run_f64
+    ref = run(d, og, mask, a.copy(), a.copy(), 2 * d.tsteps)
+    block = (64, d.max_threads // 64, 1)
+    nblk = tuple((geo.threads[i] + block[i] - 1) // block[i] for i in range(3))
+    assert nblk[0] > 1 and nblk[1] > 1
+    got, bad = emulate.run(d, geo, mask.astype(dt), a.copy(), a.copy(), 2, block, nblk)
+    assert bad == 0, f"{bad} LDS accesses outside the tile buffers"
+    ui = np.uint32 if dt == np.float32 else np.uint64
+    diff = np.flatnonzero(got.view(ui) != ref.view(ui))
+    assert diff.size == 0, f"{diff.size} points differ, first {diff[:5]}"
+
+
+def test_temporal_descriptor_limits():
+    T = spec.TEMPORAL
+    with pytest.raises(ValueError, match="2D"):
+        StencilDesc(dimension=3, variant=T, tsteps=2)
+    with pytest.raises(ValueError, match="LDS"):
+        StencilDesc(dimension=2, variant=T, range=4, xc=4, yc=8, tsteps=4, max_threads=256)
+    with pytest.raises(ValueError, match="temporal variant"):
+        StencilDesc(dimension=2, variant=spec.PLAIN, tsteps=2)
+    with pytest.raises(ValueError, match="code-shape"):
+        StencilDesc(dimension=2, variant=T, tsteps=2, xcd_remap=True)
+    d = StencilDesc(dimension=2, neighbourhood=BOX, range=1, yc=8, variant=T, tsteps=4, max_threads=256)
+    assert d.tile == (64, 32, 72, 40) and d.lds_bytes == 2 * 72 * 40 * 8
+    assert d.params_blob()[13] == 2 and d.params_blob()[18] == 4 << 8

@@ -1,0 +1,92 @@
+"""Temporal kernels on the GPU: each launch advances tsteps time steps with the
+tile and its tsteps*r halo staged in LDS (no reference counterpart; the
+reference runs one step per launch, hip/generateLaunchUpdateGrid.py:129-139).
+k launches equal k*tsteps steps of the oracle bit for bit, on the small-grid
+matrix and on the 16384^2 2D Box r=1 workload, and equal k*tsteps launches of
+the one-step kernel of the same stencil."""
+import numpy as np
+import pytest
+
+import oracle
+from threadcoarsenedstencilgenerator_amd import manifest, spec, tuned
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(d, size, launches, mode=1, seed=4, halo=0.375):
+    """(final grid from the GPU, oracle's after launches*tsteps steps)."""
+    import torch
+
+    from threadcoarsenedstencilgenerator_amd import runtime as rt
+    k = rt.Kernel(d)
+    assert k.steps_per_launch == d.tsteps
+    mask = oracle.mask_seeded(6, d.n)
+    if d.coeff == spec.VARIABLE:
+        k.set_mask(mask)
+    g = k.geometry(size)
+    og = oracle.Geometry(2, d.range, d.alignment, d.cl_size, size, (1, 1, 1) if d.remainder else d.coarsening)
+    assert (og.xdim, og.ydim) == (g.size.x, g.size.y)
+    a = np.zeros(og.n_points)
+    oracle.init_seeded(og, mode, seed, a, halo=halo)
+    dt = np.float32 if d.dtype == spec.F32C else np.float64
+    a = a.astype(dt)
+    grids = rt.Grids(g, k.dtype)
+    grids.upload(0, a)
+    grids.upload(1, a)
+    block = (64, d.max_threads // 64, 1)
+    fin = k.run(g, block, rt.grid_blocks(g, block), grids.ptr(0), grids.ptr(1), launches)
+    torch.cuda.synchronize()
+    got = grids.host(fin)
+    run = oracle.run_f32c if d.dtype == spec.F32C else oracle.run_f64
+    ref = run(d, og, mask, a.copy(), a.copy(), launches * d.tsteps)
+    del grids
+    k.close()
+    return got, ref
+
+
+@pytest.mark.parametrize("case", manifest.temporal_cases(), ids=[c[0].key() for c in manifest.temporal_cases()])
+def test_temporal_kernel_matches_oracle(case):
+    d, size = case
+    print(f"temporal {d.key()} size {size}", flush=True)
+    got, ref = _run(d, size, 3)
+    ui = np.uint32 if d.dtype == spec.F32C else np.uint64
+    bad = np.flatnonzero(got.view(ui) != ref.view(ui))
+    assert bad.size == 0, f"{bad.size} points differ, first {bad[:5]}"
+
+
+def test_temporal_refuses_other_block_shapes():
+    from threadcoarsenedstencilgenerator_amd import runtime as rt
+    d, _ = manifest.temporal_cases()[1]
+    k = rt.Kernel(d)
+    assert k.block_ok((64, d.max_threads // 64, 1))
+    assert not k.block_ok((128, d.max_threads // 128, 1)) and not k.block_ok((64, 1, 1))
+    k.close()
+
+
+def test_temporal_workload_matches_oracle_and_one_step_kernel():
+    """The tuned temporal kernel on the 16384^2 2D Box Jacobi r=1 grid: two
+    launches equal the oracle and the one-step kernel bit for bit."""
+    import torch
+
+    from threadcoarsenedstencilgenerator_amd import runtime as rt
+    cfg = tuned.BOX2D_R1_TEMPORAL
+    d = cfg["desc"]
+    size = tuned.BOX2D_R1_GRID
+    print(f"temporal workload {d.key()} size {size}", flush=True)
+    got, ref = _run(d, size, 2, mode=0, seed=9, halo=0.0)
+    assert np.array_equal(got.view(np.uint64), ref.view(np.uint64))
+    one = tuned.BOX2D_R1_ONE_STEP
+    k = rt.Kernel(one["desc"])
+    g = k.geometry(size)
+    grids = rt.Grids(g, k.dtype)
+    og = oracle.Geometry(2, 1, 0, 0, size, (1, 1, 1))
+    a = np.zeros(og.n_points)
+    oracle.init_seeded(og, 0, 9, a)
+    grids.upload(0, a)
+    grids.upload(1, a)
+    b = tuple(one["block"])
+    fin = k.run(g, b, rt.grid_blocks(g, b), grids.ptr(0), grids.ptr(1), 2 * d.tsteps)
+    torch.cuda.synchronize()
+    assert np.array_equal(grids.host(fin).view(np.uint64), got.view(np.uint64))
+    del grids
+    k.close()

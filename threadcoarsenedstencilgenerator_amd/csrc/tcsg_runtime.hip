@@ -56,7 +56,8 @@ size_t elem_size(const tcsg_desc &d) { return d.dtype == TCSG_F32C ? 4 : 8; }
 bool desc_valid(const tcsg_desc &d)
 {
     if (d.dimension != 2 && d.dimension != 3) return false;
-    if (d.neighbourhood > 1 || d.orientation > 1 || d.coeff > 1 || d.dtype > 1 || d.variant > 1) return false;
+    if (d.neighbourhood > 1 || d.orientation > 1 || d.coeff > 1 || d.dtype > 1 || d.variant > 2) return false;
+    if (d.variant == TCSG_TEMPORAL && d.dimension != 2) return false;
     if (d.range < 1 || d.xc < 1 || d.yc < 1 || d.zc < 1) return false;
     if (d.dimension == 2 && d.zc != 1) return false;
     if (d.alignment > 2) return false;
@@ -201,7 +202,9 @@ int geometry_compute(const tcsg_desc &d, tcsg_dim3 size, tcsg_geometry &g, bool 
         // remainder kernels shift the last thread of an axis back by less than
         // one coarsening (y and z; x too in PLAIN kernels), which needs an
         // interior of at least one
-        if (remainder && (a > 0 || d.variant == TCSG_PLAIN) && e[a] < c[a]) return TCSG_ERR_ARG;
+        // (temporal kernels test every point against the interior instead)
+        if (remainder && (a > 0 || d.variant == TCSG_PLAIN) && d.variant != TCSG_TEMPORAL && e[a] < c[a])
+            return TCSG_ERR_ARG;
     }
     uint64_t xmem = t[0], lo = 0, ro = 0;
     if (d.alignment == TCSG_ALIGN_READ) {
@@ -241,6 +244,7 @@ struct tcsg_kernel {
     int legal_max = 0;
     int xext_mode = 0;    // STREAM x-neighbour exchange (params[16]): 0 loads, 1 DPP, 2 wave-aligned DPP
     bool remainder = false;  // kernel generated with remainder handling (params[18] bit 2)
+    uint32_t tsteps = 1;     // time steps one launch advances (temporal kernels: params[18] bits 8-15)
 };
 
 extern "C" {
@@ -324,6 +328,8 @@ int tcsg_kernel_load(tcsg_kernel **out, const char *path, const tcsg_desc *expec
     k->xext_mode = d.variant == TCSG_STREAM ? (int)q[16] : 0;
     if (k->xext_mode > 2) return fail(TCSG_ERR_MISMATCH);
     k->remainder = (q[18] & 4u) != 0;
+    k->tsteps = d.variant == TCSG_TEMPORAL ? (q[18] >> 8) & 0xFFu : 1u;
+    if (k->tsteps < 1 || (d.variant == TCSG_TEMPORAL && q[15] % 64)) return fail(TCSG_ERR_MISMATCH);
     *out = k;
     return TCSG_OK;
 }
@@ -395,6 +401,9 @@ int tcsg_check_block(const tcsg_kernel *k, tcsg_dim3 b)
     const uint64_t t = (uint64_t)b.x * b.y * b.z;
     if (t > (uint64_t)k->legal_max) return TCSG_ERR_BLOCK_SHAPE;
     if (k->xext_mode == 2 && b.x % 64 != 0) return TCSG_ERR_BLOCK_SHAPE;  // block rows must be whole waves
+    // a temporal kernel's LDS tile is sized for its compiled block (64, launch bound / 64, 1)
+    if (k->desc.variant == TCSG_TEMPORAL && (b.x != 64 || b.y != k->params[15] / 64 || b.z != 1))
+        return TCSG_ERR_BLOCK_SHAPE;
     if (k->xext_mode == 1) {
         // every run of lanes in one block row and one wave must span the
         // H = ceil(r/xc) lanes the x-neighbour exchange reaches
@@ -444,6 +453,13 @@ int tcsg_kernel_geometry(const tcsg_kernel *k, tcsg_dim3 size, tcsg_geometry *ou
 {
     if (!k || !out) return TCSG_ERR_ARG;
     return geometry_compute(k->desc, size, *out, k->remainder);
+}
+
+int tcsg_kernel_steps_per_launch(const tcsg_kernel *k, int *out)
+{
+    if (!k || !out) return TCSG_ERR_ARG;
+    *out = (int)k->tsteps;
+    return TCSG_OK;
 }
 
 int tcsg_kernel_remainder(const tcsg_kernel *k, int *out)

@@ -9,7 +9,8 @@ hip/generateLaunchUpdateGrid.py:24-45.)  The kernel generators print gfx950
 HIP source on stdout, as the reference prints kernels.cu; `-o FILE.co`
 compiles it to a code object instead.  Options that select the MI355X code
 shape (no reference counterpart) are given as --name=value:
---variant=plain|stream, --xext=load|dpp|dppw, --dtype=f64|f32c,
+--variant=plain|stream|temporal, --xext=load|dpp|dppw, --dtype=f64|f32c,
+--tsteps=N (temporal variant, 2D: N time steps per launch through LDS),
 --max-threads=N, --prefetch=N, --waves-per-eu=N, --comp-group=N,
 --order=point|offset (plain variant), --no-xcd-remap, --no-nt-store, --zfirst,
 --remainder (grids whose interior is not a multiple of the coarsening are
@@ -102,11 +103,15 @@ def desc_from_argv(dim: int, argv):
               xext=opts.get("xext", "dpp"), xcd_remap="no-xcd-remap" not in opts, nt_store="no-nt-store" not in opts,
               zfirst="zfirst" in opts, remainder="remainder" in opts, edge_pairs="edge-pairs" in opts)
     for k, f in (("max-threads", "max_threads"), ("prefetch", "prefetch"), ("waves-per-eu", "waves_per_eu"),
-                 ("comp-group", "comp_group")):
+                 ("comp-group", "comp_group"), ("tsteps", "tsteps")):
         if k in opts:
             kw[f] = _int(opts[k], k, usage)
     if kw["variant"] == spec.PLAIN:
         kw.update(xext="load", xcd_remap=False, nt_store=False, zfirst=False, order=opts.get("order", "point"))
+    elif kw["variant"] == spec.TEMPORAL:
+        kw.update(xext="load", xcd_remap=False, nt_store=False)
+        if "order" in opts:
+            _err("--order applies to the plain variant", usage)
     elif "order" in opts:
         _err("--order applies to the plain variant", usage)
     try:
@@ -128,7 +133,7 @@ def generate(dim: int, argv=None) -> int:
             _run([HIPCC, *HIP_FLAGS, "--genco", "--no-gpu-bundle-output", src, "-o", out])
     else:
         # the printed kernels.cu also carries the reference kernel file's symbols (FP64)
-        sys.stdout.write(emit_source(d, plugin=d.dtype == spec.F64))
+        sys.stdout.write(emit_source(d, plugin=d.dtype == spec.F64 and d.variant != spec.TEMPORAL))
     return 0
 
 

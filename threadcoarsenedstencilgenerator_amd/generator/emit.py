@@ -50,7 +50,7 @@ from collections import defaultdict
 
 from .. import spec
 from ..spec import (ALIGN_NONE, ALIGN_WRITE, BOX, F32C, F64, PLAIN, ROW,
-                    STREAM, VARIABLE, StencilDesc)
+                    STREAM, TEMPORAL, VARIABLE, StencilDesc)
 
 # Byte offset given to lanes outside the interior.  The runtime only launches a
 # STREAM kernel when buffer bytes <= OOB_BASE - (largest negative footprint
@@ -779,8 +779,211 @@ class _Emitter:
                             w(f"  tcsg_st(rout, spart{c}, {soff(j, kk)}, {row[c]});")
 
     # ------------------------------------------------------------------ all
+    # ------------------------------------------------------------- TEMPORAL
+    def source_temporal(self) -> str:
+        """2D kernel that advances tsteps time steps per launch.  Workgroup
+        (64, BY) owns an output tile of TX = 64 xc by TY = BY yc points; it
+        loads the tile plus H = tsteps*r points per side into LDS, then each
+        sub-step s updates the region s*r inside the LDS tile from the previous
+        sub-step's values (ping-pong between two LDS buffers), and the last
+        sub-step's tile goes to the output.  Points outside the interior (the
+        Dirichlet halo) keep their values; points past the allocation are read
+        as 0 and are never inputs of an interior point.  Every interior point
+        sums its terms in the reference's order with the same operations as a
+        one-step kernel, so k launches equal k*tsteps steps bit for bit."""
+        d = self.d
+        T = self.T
+        w = self.w
+        cpu = self.target == "cpu"
+        R, LO, K = self.R, d.left_offset, d.tsteps
+        H = K * R
+        BY = d.max_threads // 64
+        TX, TY, LW, LH = d.tile
+        nbname = "Box" if d.neighbourhood == BOX else "Star"
+        w("// Generated by threadcoarsenedstencilgenerator_amd.generator -- do not edit.")
+        w(f"// 2D {nbname} r={R} {'Variable' if d.coeff == VARIABLE else 'Jacobi'} "
+          f"{'Row' if d.orientation == ROW else 'Column'} alignment {['None', 'Read', 'Write'][d.alignment]}/"
+          f"{d.cl_size} {'f32c' if self.f32 else 'f64'} temporal: {K} steps per launch, tile {TX}x{TY}, "
+          f"block 64x{BY}" + (" (CPU emulation build)" if cpu else ""))
+        fn = "static inline" if cpu else "__device__ __forceinline__"
+        if cpu:
+            w("#include <cmath>")
+            w("#include <cstddef>")
+            w("#include <cstdint>")
+            w("#include <vector>")
+            w("typedef uint32_t u32;")
+            w(f"typedef {T} TV;")
+            w("static inline double tcsg_fma(double a, double b, double c) { return std::fma(a, b, c); }")
+            w("static inline float tcsg_fma(float a, float b, float c) { return std::fmaf(a, b, c); }")
+        else:
+            w("#include <hip/hip_runtime.h>")
+            w("#include <stdint.h>")
+            w("// sums are written with explicit fma; no other product may be fused into an add,")
+            w("// whatever -ffp-contract the including build uses")
+            w("#pragma clang fp contract(off)")
+            w("typedef uint32_t u32;")
+            w(f"typedef {T} TV;")
+            w("__device__ __forceinline__ double tcsg_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }")
+            w("__device__ __forceinline__ float tcsg_fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }")
+        if self.f32:
+            w(f"template <class F> {fn} void tcsg_two_sum(F a, F b, F& s, F& e) {{")
+            w("  const F x = a + b;")
+            w("  const F bp = x - a;")
+            w("  const F ap = x - bp;")
+            w("  e = (a - ap) + (b - bp);")
+            w("  s = x;")
+            w("}")
+        w("")
+        w("struct tcsg_kargs {")
+        w(f"  const {T}* in;")
+        w(f"  {T}* out;")
+        w(f"  const {T}* mask;")
+        w("  uint32_t xdim, ydim, zdim, right_offset;")
+        w("  uint32_t in_bytes, out_bytes, pad0, pad1;")
+        w("};")
+        blob = ", ".join(str(v) for v in d.params_blob())
+        w(f'extern "C" {"" if cpu else "__device__ "}const uint32_t tcsg_params[24] = {{{blob}}};')
+        if not cpu:
+            w(f"__attribute__((unused)) static const uint32_t tcsg_params_host[24] = {{{blob}}};")
+        w("")
+        if cpu:
+            w("// LDS accesses are range-checked here: one outside the two tile buffers is counted")
+            w("// (and reads a dummy) instead of touching other memory")
+            w('extern "C" { unsigned long tcsg_emu_bad_access = 0; }')
+            w("static TV* tcsg_lds_lo = nullptr;")
+            w("static TV tcsg_lds_dummy;")
+            w(f"static inline TV& tcsg_lds(TV* p) {{")
+            w(f"  if (p < tcsg_lds_lo || p >= tcsg_lds_lo + 2 * {LW * LH}) {{ ++tcsg_emu_bad_access; return tcsg_lds_dummy; }}")
+            w("  return *p;")
+            w("}")
+            w("#define TCSG_LDS(p, k) tcsg_lds(const_cast<TV*>(p) + (k))")
+        else:
+            w("#define TCSG_LDS(p, k) (p)[k]")
+        w(f"#define TCSG_LW {LW}")
+        w(f"#define TCSG_LH {LH}")
+        w(f"#define TCSG_BUF {LW * LH}")
+        w("// interior of the padded grid: x in [r + left offset, xdim - right offset - r), y in [r, ydim - r)")
+        w(f"{fn} bool tcsg_interior(const tcsg_kargs& a, long long gx, long long gy) {{")
+        w(f"  return gx >= {R + LO} && gx < (long long)a.xdim - (long long)a.right_offset - {R} &&")
+        w(f"         gy >= {R} && gy < (long long)a.ydim - {R};")
+        w("}")
+        w("// global x and y of the LDS tile's first column and row")
+        w(f"{fn} long long tcsg_x0(u32 bx) {{ return (long long)bx * {TX} + {R + LO - H}; }}")
+        w(f"{fn} long long tcsg_y0(u32 by) {{ return (long long)by * {TY} + {R - H}; }}")
+        w("")
+        w("// the tile and its halo from the input grid into LDS buffer 0")
+        w(f"{fn} void tcsg_load(const tcsg_kargs& a, TV* tb, u32 bx, u32 by, u32 tx, u32 ty) {{")
+        w("  const long long x0 = tcsg_x0(bx), y0 = tcsg_y0(by);")
+        w(f"  for (u32 j = ty; j < TCSG_LH; j += {BY}u) {{")
+        w("    const long long gy = y0 + j;")
+        w("    const bool rowok = gy >= 0 && gy < (long long)a.ydim;")
+        w("    for (u32 i = tx; i < TCSG_LW; i += 64u) {")
+        w("      const long long gx = x0 + i;")
+        w(f"      TV v = ({T})0;")
+        w("      if (rowok && gx >= 0 && gx < (long long)a.xdim) v = a.in[(size_t)gy * a.xdim + (size_t)gx];")
+        w("      TCSG_LDS(tb, j * TCSG_LW + i) = v;")
+        w("    }")
+        w("  }")
+        w("}")
+        w("")
+        w("// sub-step s (1..tsteps): the region s*r inside the tile, from buffer (s-1)&1 into s&1.")
+        w("// Each wave walks a strip of consecutive rows down its columns, keeping the rows")
+        w("// above and below the current one in registers: per point it reads from LDS only")
+        w("// the row entering the window (and, for a Star, the current row's x-neighbours).")
+        w(f"{fn} void tcsg_substep(const tcsg_kargs& a, TV* tb, u32 bx, u32 by, u32 tx, u32 ty, u32 s) {{")
+        w("  const long long x0 = tcsg_x0(bx), y0 = tcsg_y0(by);")
+        w("  const TV* cur = tb + ((s - 1u) & 1u) * TCSG_BUF;")
+        w("  TV* nxt = tb + (s & 1u) * TCSG_BUF;")
+        w(f"  const u32 m = s * {R}u;")
+        w(f"  const u32 rows = (TCSG_LH - 2u * m + {BY - 1}u) / {BY}u;")
+        w("  const u32 ja = m + ty * rows;")
+        w("  const u32 je = ja + rows < TCSG_LH - m ? ja + rows : TCSG_LH - m;")
+        if d.coeff == VARIABLE:
+            w(f"  const {T}* mk = a.mask;")
+            for mi in range(d.n):
+                w(f"  const TV w{mi} = mk[{mi}];")
+        box = d.neighbourhood == BOX
+        # the window: offsets (dx, dy) kept in registers; a Star keeps only its y arm there
+        win_dx = list(range(-R, R + 1)) if box else [0]
+        vn = lambda dx, dy: f"u_{_n(dx)}_{_n(dy)}"
+        w("  for (u32 i = m + tx; i < TCSG_LW - m; i += 64u) {")
+        w("    const TV* col = cur + i;  // signed offsets from here: the window reaches r columns left")
+        w("    const long long gx = x0 + i;")
+        w("    if (ja >= je) break;")
+        # rows ja-R .. ja+R-1 enter the window before the walk
+        for dy in range(-R, R):
+            for dx in win_dx:
+                w(f"    TV {vn(dx, dy)} = TCSG_LDS(col, (int)(ja * TCSG_LW) + ({dy * LW + dx}));")
+        w("    for (u32 j = ja; j < je; ++j) {")
+        for dx in win_dx:
+            w(f"      const TV {vn(dx, R)} = TCSG_LDS(col, (int)(j * TCSG_LW) + ({R * LW + dx}));")
+        if not box:
+            for dx in range(-R, R + 1):
+                if dx:
+                    w(f"      const TV {vn(dx, 0)} = TCSG_LDS(col, (int)(j * TCSG_LW) + ({dx}));")
+        w(f"      TV res = {vn(0, 0)};")
+        w("      if (tcsg_interior(a, gx, y0 + j)) {")
+        o = (0, 0, 0)
+        for t, (dx, dy, dz) in enumerate(self.offs):
+            mi = spec.mask_index(d, dx, dy, dz)
+            w("        " + self.acc_stmt(o, t, mi, vn(dx, dy)))
+        w(f"        res = {self.final_expr(o)};")
+        w("      }")
+        w("      TCSG_LDS(nxt, j * TCSG_LW + i) = res;")
+        for dy in range(-R, R):
+            for dx in win_dx:
+                w(f"      {vn(dx, dy)} = {vn(dx, dy + 1)};")
+        w("    }")
+        w("  }")
+        w("}")
+        w("")
+        w("// the tile's interior points of the last sub-step to the output grid")
+        w(f"{fn} void tcsg_store(const tcsg_kargs& a, const TV* tb, u32 bx, u32 by, u32 tx, u32 ty) {{")
+        w("  const long long x0 = tcsg_x0(bx), y0 = tcsg_y0(by);")
+        w(f"  const TV* fin = tb + {K & 1}u * TCSG_BUF;")
+        w(f"  for (u32 j = {H}u + ty; j < {H + TY}u; j += {BY}u) {{")
+        w(f"    for (u32 i = {H}u + tx; i < {H + TX}u; i += 64u) {{")
+        w("      if (tcsg_interior(a, x0 + i, y0 + j))")
+        w("        a.out[(size_t)(y0 + j) * a.xdim + (size_t)(x0 + i)] = TCSG_LDS(fin, j * TCSG_LW + i);")
+        w("    }")
+        w("  }")
+        w("}")
+        w("")
+        if cpu:
+            w("// One launch: the workgroups one after another; within a workgroup each phase runs for")
+            w("// every thread before the next phase starts (the kernel's barriers).")
+            w('extern "C" int tcsg_emulate(const tcsg_kargs* a, uint32_t gx, uint32_t gy, uint32_t gz,')
+            w("                            uint32_t bx, uint32_t by, uint32_t bz) {")
+            w(f"  if (bx != 64u || by != {BY}u || bz != 1u || gz != 1u) return -1;")
+            w("  std::vector<TV> tb(2 * TCSG_BUF);")
+            w("  tcsg_lds_lo = tb.data();")
+            w("  for (uint32_t wy = 0; wy < gy; ++wy) for (uint32_t wx = 0; wx < gx; ++wx) {")
+            w(f"    for (u32 t = 0; t < {64 * BY}u; ++t) tcsg_load(*a, tb.data(), wx, wy, t % 64u, t / 64u);")
+            w(f"    for (u32 s = 1; s <= {K}u; ++s)")
+            w(f"      for (u32 t = 0; t < {64 * BY}u; ++t) tcsg_substep(*a, tb.data(), wx, wy, t % 64u, t / 64u, s);")
+            w(f"    for (u32 t = 0; t < {64 * BY}u; ++t) tcsg_store(*a, tb.data(), wx, wy, t % 64u, t / 64u);")
+            w("  }")
+            w("  return 0;")
+            w("}")
+        else:
+            w(f'extern "C" __global__ void __launch_bounds__({d.max_threads}) tcsg_update(const tcsg_kargs a)')
+            w("{")
+            w("  __shared__ TV tb[2 * TCSG_BUF];")
+            w("  const u32 tx = threadIdx.x, ty = threadIdx.y;")
+            w("  tcsg_load(a, tb, blockIdx.x, blockIdx.y, tx, ty);")
+            w("  __syncthreads();")
+            w(f"  for (u32 s = 1; s <= {K}u; ++s) {{")
+            w("    tcsg_substep(a, tb, blockIdx.x, blockIdx.y, tx, ty, s);")
+            w("    __syncthreads();")
+            w("  }")
+            w("  tcsg_store(a, tb, blockIdx.x, blockIdx.y, tx, ty);")
+            w("}")
+        return "\n".join(self.L) + "\n"
+
     def source(self, plugin: bool = False) -> str:
         d = self.d
+        if d.variant == TEMPORAL:
+            return self.source_temporal()
         self.preamble()
         attr = ""
         if d.variant == STREAM and d.waves_per_eu and self.target != "cpu":
@@ -1021,8 +1224,10 @@ def emit_source(desc: StencilDesc, target: str = "gfx950", plugin: bool = False)
     if terms_per_thread(desc) > MAX_TERMS_PER_THREAD:
         raise ValueError(f"coarsening {desc.coarsening} x stencil size {desc.n} = {terms_per_thread(desc)} "
                          f"accumulate statements per thread exceeds {MAX_TERMS_PER_THREAD}")
-    if target == "cpu" and desc.variant != STREAM:
-        raise ValueError("CPU emulation covers the STREAM variant")
+    if target == "cpu" and desc.variant == PLAIN:
+        raise ValueError("CPU emulation covers the STREAM and TEMPORAL variants")
+    if plugin and desc.variant == TEMPORAL:
+        raise ValueError("the reference's updateGrid advances one step per launch; a temporal kernel has no such symbol")
     if target not in ("gfx950", "cpu"):
         raise ValueError(f"unknown target {target}")
     return _Emitter(desc, target).source(plugin)

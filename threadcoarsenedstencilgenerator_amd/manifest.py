@@ -9,7 +9,7 @@ from __future__ import annotations
 import itertools
 
 from .spec import (ALIGN_NONE, ALIGN_READ, ALIGN_WRITE, BOX, COLUMN, F32C, F64, JACOBI, PLAIN, ROW,
-                   STAR, STREAM, VARIABLE, StencilDesc)
+                   STAR, STREAM, TEMPORAL, VARIABLE, StencilDesc)
 
 # code shapes every parity case is built in
 SHAPES = {
@@ -89,6 +89,29 @@ def remainder_cases():
 
 
 # ---- benchmark workloads ----
+def temporal_cases():
+    """Temporal kernels (tsteps per launch through LDS) with a small grid each:
+    both neighbourhoods, coefficient kinds, orientations, dtypes and
+    alignments; ranges 1-3; 1-4 steps per launch; grids of several tiles whose
+    interiors are not multiples of the tile, with and without the reference's
+    truncation."""
+    base = dict(dimension=2, variant=TEMPORAL)
+    return [
+        (StencilDesc(neighbourhood=BOX, coeff=JACOBI, range=1, xc=1, yc=2, tsteps=3, max_threads=128,
+                     remainder=True, **base), (150, 41)),
+        (StencilDesc(neighbourhood=STAR, coeff=VARIABLE, range=2, xc=2, yc=1, tsteps=2, max_threads=256,
+                     alignment=ALIGN_WRITE, cl_size=8, remainder=True, **base), (301, 23)),
+        (StencilDesc(neighbourhood=BOX, orientation=COLUMN, coeff=VARIABLE, range=3, xc=1, yc=1, tsteps=4,
+                     max_threads=64, alignment=ALIGN_READ, cl_size=16, **base), (97, 30)),
+        (StencilDesc(neighbourhood=STAR, orientation=COLUMN, coeff=JACOBI, range=1, xc=1, yc=4, tsteps=1,
+                     max_threads=192, remainder=True, **base), (70, 57)),
+        (StencilDesc(neighbourhood=BOX, coeff=VARIABLE, range=2, xc=1, yc=3, tsteps=3, max_threads=128,
+                     dtype=F32C, comp_group=7, remainder=True, **base), (140, 37)),
+        (StencilDesc(neighbourhood=STAR, coeff=JACOBI, range=1, xc=2, yc=2, tsteps=4, max_threads=256,
+                     dtype=F32C, **base), (260, 45)),
+    ]
+
+
 def cfg1_desc(shape="plain"):
     """2D Box Jacobi r=1, coarsening 1x1, Row, alignment None."""
     return StencilDesc(dimension=2, neighbourhood=BOX, orientation=ROW, coeff=JACOBI, range=1, xc=1, yc=1,
@@ -160,6 +183,7 @@ def default_build_set():
     (cfg2_sweep, cfg2_tuned, cfg2_f32c_candidates, ...) are compiled on
     demand by the sweeps and cached in the same directory."""
     descs = [d for d, _ in parity_cases()] + [d for d, _ in f32c_cases()] + [d for d, _ in remainder_cases()]
+    descs += [d for d, _ in temporal_cases()]
     descs += [cfg1_desc("plain"), cfg1_desc("stream"), cfg1_desc("dppw")]
     descs += [cfg2_desc(1, 1, 1), cfg2_desc(2, 4, 8)]            # reference-shaped kernels
     descs += [smoke_desc()]

@@ -81,6 +81,7 @@ def _lib():
         L.tcsg_kernel_desc.argtypes = [p, C.POINTER(CDesc)]
         L.tcsg_kernel_geometry.argtypes = [p, Dim3, C.POINTER(CGeometry)]
         L.tcsg_kernel_remainder.argtypes = [p, C.POINTER(i)]
+        L.tcsg_kernel_steps_per_launch.argtypes = [p, C.POINTER(i)]
         L.tcsg_set_mask.argtypes = [p, p, u32]
         L.tcsg_max_block_size.argtypes = [p, C.POINTER(i), C.POINTER(i)]
         L.tcsg_check_block.argtypes = [p, Dim3]
@@ -233,6 +234,13 @@ class Kernel:
         v = C.c_int()
         _check(_lib().tcsg_kernel_remainder(self._h, C.byref(v)), "kernel_remainder")
         return bool(v.value)
+
+    @property
+    def steps_per_launch(self) -> int:
+        """Time steps one launch advances (temporal kernels: their tsteps)."""
+        v = C.c_int()
+        _check(_lib().tcsg_kernel_steps_per_launch(self._h, C.byref(v)), "kernel_steps_per_launch")
+        return v.value
 
     def init(self, g: CGeometry, ptr: int, mode=INIT_REFERENCE, seed=1, stream=None):
         """launchInit (hip/generateLaunchUpdateGrid.py:150-158) and seeded fills."""

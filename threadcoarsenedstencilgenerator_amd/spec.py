@@ -15,14 +15,14 @@ ROW, COLUMN = 0, 1
 JACOBI, VARIABLE = 0, 1
 ALIGN_NONE, ALIGN_READ, ALIGN_WRITE = 0, 1, 2
 F64, F32C = 0, 1
-PLAIN, STREAM = 0, 1
+PLAIN, STREAM, TEMPORAL = 0, 1, 2
 
 NB_NAMES = {"Box": BOX, "Star": STAR}
 ORIENT_NAMES = {"Row": ROW, "Column": COLUMN}
 COEFF_NAMES = {"Jacobi": JACOBI, "Variable": VARIABLE}
 ALIGN_NAMES = {"None": ALIGN_NONE, "Read": ALIGN_READ, "Write": ALIGN_WRITE}
 DTYPE_NAMES = {"f64": F64, "f32c": F32C}
-VARIANT_NAMES = {"plain": PLAIN, "stream": STREAM}
+VARIANT_NAMES = {"plain": PLAIN, "stream": STREAM, "temporal": TEMPORAL}
 
 
 def _inv(d):
@@ -36,8 +36,12 @@ class StencilDesc:
     dimension/neighbourhood/orientation/coeff/range/coarsening/alignment/
     cl_size mirror the reference generator argv.  dtype, variant and the
     tuning knobs (max_threads, xext, xcd_remap, nt_store, comp_group,
-    prefetch, waves_per_eu, zfirst, batched, nt_load, order, remainder) have no reference counterpart: they select the
-    MI355X code shape.
+    prefetch, waves_per_eu, zfirst, batched, nt_load, order, remainder,
+    edge_pairs, tsteps) have no reference counterpart: they select the MI355X
+    code shape.  The TEMPORAL variant (2D) advances tsteps time steps per
+    launch: each workgroup stages its tile of (64 xc) x (max_threads/64 yc)
+    points plus tsteps*range halo points per side in LDS and runs the steps
+    there.
     """
 
     dimension: int = 3
@@ -73,6 +77,7 @@ class StencilDesc:
     #                          reference does
     edge_pairs: bool = False  # dppw: the edge columns of two adjacent rows in one load (the
     #                          second row's in the next edge lanes, moved by a DPP row shift)
+    tsteps: int = 1          # TEMPORAL: time steps per launch
     order: str = "point"     # PLAIN statement order: "point" loads each footprint point once and
     #                          adds it to every output that uses it; "offset" walks the mask in
     #                          summation order, every output per offset (weights used in order,
@@ -116,6 +121,20 @@ class StencilDesc:
             raise ValueError("FP32-storage STREAM kernels take at most 2 points per lane in x")
         if self.edge_pairs and (self.variant != STREAM or self.xext != "dppw"):
             raise ValueError("edge_pairs is an option of the wave-aligned DPP code shape")
+        if self.variant not in (PLAIN, STREAM, TEMPORAL):
+            raise ValueError("unknown code variant")
+        if self.variant != TEMPORAL and self.tsteps != 1:
+            raise ValueError("tsteps > 1 needs the temporal variant")
+        if self.variant == TEMPORAL:
+            if self.dimension != 2:
+                raise ValueError("the temporal variant is built for 2D stencils")
+            if not 1 <= self.tsteps <= 16:
+                raise ValueError("tsteps must be in [1, 16]")
+            if (self.xext != "load" or self.xcd_remap or self.nt_store or self.nt_load or self.zfirst
+                    or self.batched or self.order != "point"):
+                raise ValueError("the temporal variant takes none of the STREAM/PLAIN code-shape options")
+            if self.lds_bytes > 65536:
+                raise ValueError(f"temporal tile needs {self.lds_bytes} bytes of LDS (at most 65536)")
 
     # ---- derived quantities ----
     @property
@@ -140,6 +159,19 @@ class StencilDesc:
         return 0
 
     @property
+    def tile(self):
+        """TEMPORAL: output points per workgroup in x and y, and the LDS tile
+        (outputs plus tsteps*range halo points per side)."""
+        h = self.tsteps * self.range
+        tx, ty = 64 * self.xc, self.max_threads // 64 * self.yc
+        return tx, ty, tx + 2 * h, ty + 2 * h
+
+    @property
+    def lds_bytes(self) -> int:
+        _, _, lw, lh = self.tile
+        return 2 * lw * lh * (4 if self.dtype == F32C else 8)
+
+    @property
     def coarsening(self):
         return (self.xc, self.yc, self.zc)
 
@@ -155,7 +187,8 @@ class StencilDesc:
                 f"_{_inv(ORIENT_NAMES)[self.orientation].lower()}_c{self.xc}x{self.yc}x{self.zc}"
                 f"_{_inv(ALIGN_NAMES)[self.alignment].lower()}{self.cl_size}"
                 f"_{_inv(DTYPE_NAMES)[self.dtype]}_{_inv(VARIANT_NAMES)[self.variant]}"
-                f"{'' if self.variant == PLAIN else '_' + self.xext}_{h}")
+                f"{'' if self.variant == PLAIN else '_' + self.xext if self.variant == STREAM else f'_k{self.tsteps}'}"
+                f"_{h}")
 
     def params_blob(self):
         """Integers baked into the code object and checked by the runtime
@@ -165,7 +198,8 @@ class StencilDesc:
                 self.coeff, self.range, self.xc, self.yc, self.zc, self.alignment,
                 self.cl_size, self.dtype, self.variant, self.n, self.max_threads,
                 {"load": 0, "dpp": 1, "dppw": 2}[self.xext], int(self.xcd_remap),
-                int(self.nt_store) | int(self.nt_load) << 1 | int(self.remainder) << 2,
+                int(self.nt_store) | int(self.nt_load) << 1 | int(self.remainder) << 2
+                | (self.tsteps if self.variant == TEMPORAL else 0) << 8,
                 self.comp_group, self.prefetch, self.waves_per_eu, int(self.zfirst), int(self.batched)]
 
     def replace(self, **kw) -> "StencilDesc":
@@ -174,7 +208,7 @@ class StencilDesc:
 
 # knobs added after kernel names were first recorded (measured corpora refer to
 # kernels by name): left out of the name's hash while at their default
-_LATER_FIELDS = {"nt_load": False, "order": "point", "remainder": False, "edge_pairs": False}
+_LATER_FIELDS = {"nt_load": False, "order": "point", "remainder": False, "edge_pairs": False, "tsteps": 1}
 
 
 def mask_index(desc: StencilDesc, dx: int, dy: int, dz: int = 0):
@@ -286,7 +320,7 @@ def geometry(desc: StencilDesc, size) -> Geometry:
             raise ValueError(f"grid axis {a} ({size[a]}) too small for range {r}")
         t[a] = size[a] if desc.remainder else size[a] - (size[a] - 2 * r) % c[a]
         e[a] = t[a] - 2 * r
-        if desc.remainder and (a > 0 or desc.variant == PLAIN) and e[a] < c[a]:
+        if desc.remainder and (a > 0 or desc.variant == PLAIN) and desc.variant != TEMPORAL and e[a] < c[a]:
             raise ValueError("remainder handling needs an interior of at least one coarsening "
                              + ("per axis" if desc.variant == PLAIN else "in y and z"))
     lo = ro = 0

@@ -10,7 +10,7 @@ z-chunk, so no 128-byte line at a tile's x edge is fetched by two XCDs
 from __future__ import annotations
 
 from . import manifest
-from .spec import F32C, F64
+from .spec import F32C, F64, TEMPORAL
 
 # 3D Star Variable r=4, 512^3 interior, FP64, Write alignment at 16 points
 CFG2_F64 = dict(desc=manifest.cfg2_desc(2, 4, 128, shape="dppw", max_threads=256).replace(prefetch=0, zfirst=False),
@@ -40,6 +40,17 @@ BOX2D_R8_F32C = dict(desc=manifest.cfg3_desc(4, 4, shape="plain", max_threads=25
     order="offset", comp_group=17), block=(128, 2, 1))
 
 
+# 2D Box Jacobi r=1 (the 1024^2 workload's stencil) on a 16384x16384 interior, where it is
+# HBM-bound: the temporal kernel advances several steps per launch through LDS; the one-step
+# kernel is the fastest single-step shape measured on the same grid
+BOX2D_R1_GRID = (16384 + 2, 16384 + 2)
+# (sweep over 1-8 steps per launch and tiles of 64x16, 64x32 and 128x16 points: 6 steps on 64x32-point tiles
+# with 512 threads was fastest, 498 GLup/s against 303 for the best one-step shape)
+BOX2D_R1_TEMPORAL = dict(desc=manifest.cfg1_desc("plain").replace(
+    variant=TEMPORAL, tsteps=6, xc=1, yc=4, max_threads=512, remainder=True), block=(64, 8, 1))
+BOX2D_R1_ONE_STEP = dict(desc=manifest.cfg1_desc("dppw").replace(xc=2, yc=4, max_threads=256), block=(256, 1, 1))
+
+
 def best(workload: str, dtype: int = F64):
     if workload == "star3d_r4_512":
         return CFG2_F32C if dtype == F32C else CFG2_F64
@@ -50,4 +61,5 @@ def best(workload: str, dtype: int = F64):
 
 def build_set():
     return [CFG2_F64["desc"], CFG2_F32C["desc"], BOX2D_R8_F64["desc"], BOX2D_R8_F32C["desc"],
-            CFG2_F64["desc"].replace(remainder=True), CFG2_F32C["desc"].replace(remainder=True)]
+            CFG2_F64["desc"].replace(remainder=True), CFG2_F32C["desc"].replace(remainder=True),
+            BOX2D_R1_TEMPORAL["desc"], BOX2D_R1_ONE_STEP["desc"]]

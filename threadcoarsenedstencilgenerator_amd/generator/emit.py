@@ -871,19 +871,30 @@ class _Emitter:
         w(f"{fn} long long tcsg_x0(u32 bx) {{ return (long long)bx * {TX} + {R + LO - H}; }}")
         w(f"{fn} long long tcsg_y0(u32 by) {{ return (long long)by * {TY} + {R - H}; }}")
         w("")
-        w("// the tile and its halo from the input grid into LDS buffer 0")
+        w("// the tile and its halo from the input grid into LDS buffer 0: every load of the thread")
+        w("// is issued before the first LDS write (rows ty + k*BY, columns tx + 64c)")
         w(f"{fn} void tcsg_load(const tcsg_kargs& a, TV* tb, u32 bx, u32 by, u32 tx, u32 ty) {{")
         w("  const long long x0 = tcsg_x0(bx), y0 = tcsg_y0(by);")
-        w(f"  for (u32 j = ty; j < TCSG_LH; j += {BY}u) {{")
-        w("    const long long gy = y0 + j;")
-        w("    const bool rowok = gy >= 0 && gy < (long long)a.ydim;")
-        w("    for (u32 i = tx; i < TCSG_LW; i += 64u) {")
-        w("      const long long gx = x0 + i;")
-        w(f"      TV v = ({T})0;")
-        w("      if (rowok && gx >= 0 && gx < (long long)a.xdim) v = a.in[(size_t)gy * a.xdim + (size_t)gx];")
-        w("      TCSG_LDS(tb, j * TCSG_LW + i) = v;")
-        w("    }")
-        w("  }")
+        nr, ncol = (LH + BY - 1) // BY, (LW + 63) // 64
+        for c in range(ncol):
+            w(f"  const long long gx{c} = x0 + tx + {64 * c}u;")
+            w(f"  const bool cok{c} = {f'tx + {64 * c}u < TCSG_LW && ' if 64 * (c + 1) > LW else ''}"
+              f"gx{c} >= 0 && gx{c} < (long long)a.xdim;")
+        for r in range(nr):
+            w(f"  const long long gy{r} = y0 + ty + {r * BY}u;")
+            w(f"  const bool rok{r} = {f'ty + {r * BY}u < TCSG_LH && ' if BY * (r + 1) > LH else ''}"
+              f"gy{r} >= 0 && gy{r} < (long long)a.ydim;")
+            for c in range(ncol):
+                w(f"  const TV l{r}_{c} = rok{r} && cok{c} ? a.in[(size_t)gy{r} * a.xdim + (size_t)gx{c}] : ({T})0;")
+        for r in range(nr):
+            for c in range(ncol):
+                cond = []
+                if BY * (r + 1) > LH:
+                    cond.append(f"ty + {r * BY}u < TCSG_LH")
+                if 64 * (c + 1) > LW:
+                    cond.append(f"tx + {64 * c}u < TCSG_LW")
+                st = f"TCSG_LDS(tb, (ty + {r * BY}u) * TCSG_LW + tx + {64 * c}u) = l{r}_{c};"
+                w(f"  if ({' && '.join(cond)}) {st}" if cond else f"  {st}")
         w("}")
         w("")
         w("// sub-step s (1..tsteps): the region s*r inside the tile, from buffer (s-1)&1 into s&1.")

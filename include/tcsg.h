@@ -172,7 +172,9 @@ int tcsg_run(tcsg_kernel *k, const tcsg_geometry *g, tcsg_dim3 block, tcsg_dim3 
  * doubling rounds of 1,2,4,... steps with a device synchronisation after
  * every step and a host pointer swap, until the elapsed time exceeds
  * max_time; returns the 2^k-1 per-step durations [s] in durations (up to
- * cap; *n gets the full count).  Null stream. */
+ * cap; *n gets the full count).  Null stream.  The rounds count launches: a
+ * temporal kernel's launch advances steps_per_launch steps and contributes that
+ * many durations, each an equal share of the launch. */
 int tcsg_benchmark_update_grid(tcsg_kernel *k, const tcsg_geometry *g, tcsg_dim3 block,
                                tcsg_dim3 blocks, double max_time, void *d0, void *d1,
                                double *durations, size_t cap, size_t *n);

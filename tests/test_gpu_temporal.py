@@ -90,3 +90,25 @@ def test_temporal_workload_matches_oracle_and_one_step_kernel():
     assert np.array_equal(grids.host(fin).view(np.uint64), got.view(np.uint64))
     del grids
     k.close()
+
+
+def test_reference_timing_protocol_counts_steps():
+    """tcsg_benchmark_update_grid (the reference's doubling rounds) on a
+    temporal kernel: each launch contributes steps_per_launch per-step
+    durations, equal shares of the launch, so the reference's Lup/s per
+    duration stays a per-step rate."""
+    import torch
+
+    from threadcoarsenedstencilgenerator_amd import runtime as rt
+    d, size = manifest.temporal_cases()[2]      # 4 steps per launch
+    k = rt.Kernel(d)
+    k.set_mask(oracle.mask_seeded(6, d.n))
+    g = k.geometry(size)
+    grids = rt.Grids(g, k.dtype)
+    block = (64, d.max_threads // 64, 1)
+    durs = k.benchmark(g, block, rt.grid_blocks(g, block), 0.0, grids.ptr(0), grids.ptr(1))
+    torch.cuda.synchronize()
+    assert len(durs) == d.tsteps and d.tsteps == 4   # one round of one launch
+    assert all(x == durs[0] and x > 0 for x in durs)
+    del grids
+    k.close()

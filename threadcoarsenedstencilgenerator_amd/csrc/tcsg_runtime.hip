@@ -573,9 +573,11 @@ int tcsg_benchmark_update_grid(tcsg_kernel *k, const tcsg_geometry *g, tcsg_dim3
         }
         runtime = std::chrono::duration<double>(ts.back() - ts.front()).count();
     }
-    const size_t cnt = ts.size() - 1;
+    // per-step durations: a launch of a temporal kernel advances tsteps steps, each given an
+    // equal share of the launch's time
+    const size_t launches = ts.size() - 1, K = k->tsteps, cnt = launches * K;
     for (size_t i = 0; i < cnt && i < cap; ++i)
-        durations[i] = std::chrono::duration<double>(ts[i + 1] - ts[i]).count();
+        durations[i] = std::chrono::duration<double>(ts[i / K + 1] - ts[i / K]).count() / (double)K;
     if (n) *n = cnt;
     return TCSG_OK;
 }

@@ -914,36 +914,48 @@ class _Emitter:
             for mi in range(d.n):
                 w(f"  const TV w{mi} = mk[{mi}];")
         box = d.neighbourhood == BOX
-        # the window: offsets (dx, dy) kept in registers; a Star keeps only its y arm there
+        # the window: rows j-r..j+r in 2r+1 register slots (a Box keeps each row's 2r+1 columns,
+        # a Star only its y arm's column); row ja-r+t sits in slot t mod (2r+1), and the walk is
+        # unrolled by 2r+1 rows so that each slot keeps its registers (no moves between rows)
+        W = 2 * R + 1
         win_dx = list(range(-R, R + 1)) if box else [0]
-        vn = lambda dx, dy: f"u_{_n(dx)}_{_n(dy)}"
+        sn = lambda dx, k: f"u_{_n(dx)}_{k}"
         w("  for (u32 i = m + tx; i < TCSG_LW - m; i += 64u) {")
         w("    const TV* col = cur + i;  // signed offsets from here: the window reaches r columns left")
         w("    const long long gx = x0 + i;")
         w("    if (ja >= je) break;")
-        # rows ja-R .. ja+R-1 enter the window before the walk
-        for dy in range(-R, R):
+        for t in range(2 * R):
             for dx in win_dx:
-                w(f"    TV {vn(dx, dy)} = TCSG_LDS(col, (int)(ja * TCSG_LW) + ({dy * LW + dx}));")
-        w("    for (u32 j = ja; j < je; ++j) {")
+                w(f"    TV {sn(dx, t)} = TCSG_LDS(col, (int)(ja * TCSG_LW) + ({(t - R) * LW + dx}));")
         for dx in win_dx:
-            w(f"      const TV {vn(dx, R)} = TCSG_LDS(col, (int)(j * TCSG_LW) + ({R * LW + dx}));")
-        if not box:
-            for dx in range(-R, R + 1):
-                if dx:
-                    w(f"      const TV {vn(dx, 0)} = TCSG_LDS(col, (int)(j * TCSG_LW) + ({dx}));")
-        w(f"      TV res = {vn(0, 0)};")
-        w("      if (tcsg_interior(a, gx, y0 + j)) {")
+            w(f"    TV {sn(dx, 2 * R)};")
+        w(f"    for (u32 j0 = ja; j0 < je; j0 += {W}u) {{")
         o = (0, 0, 0)
-        for t, (dx, dy, dz) in enumerate(self.offs):
-            mi = spec.mask_index(d, dx, dy, dz)
-            w("        " + self.acc_stmt(o, t, mi, vn(dx, dy)))
-        w(f"        res = {self.final_expr(o)};")
-        w("      }")
-        w("      TCSG_LDS(nxt, j * TCSG_LW + i) = res;")
-        for dy in range(-R, R):
+        for q in range(W):
+            ind = "      "
+            w(f"{ind}if (j0 + {q}u < je) {{")
+            w(f"{ind}  const u32 j = j0 + {q}u;")
+            slot = lambda dy: (q + dy + R) % W
             for dx in win_dx:
-                w(f"      {vn(dx, dy)} = {vn(dx, dy + 1)};")
+                w(f"{ind}  {sn(dx, slot(R))} = TCSG_LDS(col, (int)(j * TCSG_LW) + ({R * LW + dx}));")
+            val = {}
+            for dy in range(-R, R + 1):
+                for dx in win_dx:
+                    val[(dx, dy)] = sn(dx, slot(dy))
+            if not box:
+                for dx in range(-R, R + 1):
+                    if dx:
+                        w(f"{ind}  const TV e_{_n(dx)} = TCSG_LDS(col, (int)(j * TCSG_LW) + ({dx}));")
+                        val[(dx, 0)] = f"e_{_n(dx)}"
+            w(f"{ind}  TV res = {val[(0, 0)]};")
+            w(f"{ind}  if (tcsg_interior(a, gx, y0 + j)) {{")
+            for t, (dx, dy, dz) in enumerate(self.offs):
+                mi = spec.mask_index(d, dx, dy, dz)
+                w(f"{ind}    " + self.acc_stmt(o, t, mi, val[(dx, dy)]))
+            w(f"{ind}    res = {self.final_expr(o)};")
+            w(f"{ind}  }}")
+            w(f"{ind}  TCSG_LDS(nxt, j * TCSG_LW + i) = res;")
+            w(f"{ind}}}")
         w("    }")
         w("  }")
         w("}")

@@ -44,10 +44,10 @@ BOX2D_R8_F32C = dict(desc=manifest.cfg3_desc(4, 4, shape="plain", max_threads=25
 # HBM-bound: the temporal kernel advances several steps per launch through LDS; the one-step
 # kernel is the fastest single-step shape measured on the same grid
 BOX2D_R1_GRID = (16384 + 2, 16384 + 2)
-# (sweep over 1-8 steps per launch and tiles of 64x16, 64x32 and 128x16 points: 6 steps on 64x32-point tiles
-# with 512 threads was fastest, 498 GLup/s against 303 for the best one-step shape)
+# (sweep over 3-8 steps per launch and tiles of 64x16, 64x32, 128x8 and 128x16 points: 4 steps on
+# 128x16-point tiles with 512 threads was fastest, 670 GLup/s against 303 for the best one-step shape)
 BOX2D_R1_TEMPORAL = dict(desc=manifest.cfg1_desc("plain").replace(
-    variant=TEMPORAL, tsteps=6, xc=1, yc=4, max_threads=512, remainder=True), block=(64, 8, 1))
+    variant=TEMPORAL, tsteps=4, xc=2, yc=2, max_threads=512, remainder=True), block=(64, 8, 1))
 BOX2D_R1_ONE_STEP = dict(desc=manifest.cfg1_desc("dppw").replace(xc=2, yc=4, max_threads=256), block=(256, 1, 1))
 
 

@@ -333,6 +333,7 @@ def bench_temporal(args):
     t, o = out["temporal"], out["one_step"]
     pts = (tuned.BOX2D_R1_GRID[0] - 2) * (tuned.BOX2D_R1_GRID[1] - 2)
     alg = 16.0 * pts / (t["kernel_ms_per_launch"] * 1e-3) / 1e9   # one read + one write per point per launch
+    tr = _traffic(tcfg["desc"].key(), tcfg["block"])
     line = {"metric": "GLup/s, FP64 2D Box Jacobi r=1 16384^2, temporal blocking", "value": t["glups"],
             "unit": "GLup/s", "n_gpus": 1, "steps": t["steps"], "warmup": args.warmup, "higher_is_better": True,
             "dtype": "f64", "data": "synthetic",
@@ -341,7 +342,8 @@ def bench_temporal(args):
                        "steps_per_launch": t["steps_per_launch"], "tile": list(tcfg["desc"].tile[:2]),
                        "block": t["block"]},
             "roofline": {"bound": "hbm", "achieved": round(alg, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": round(alg / HBM_PEAK_GBS, 4), "traffic": None,
+                         "frac": round(alg / HBM_PEAK_GBS, 4),
+                         "traffic": (tr["hbm_bytes_per_launch"] if tr else None),
                          "one_step_equivalent_gbs": round(16.0 * t["kernel_glups"], 1),
                          "note": "achieved: 16 B per point per launch (each launch advances steps_per_launch "
                                  "steps); one_step_equivalent_gbs: the HBM rate a one-step kernel would need "

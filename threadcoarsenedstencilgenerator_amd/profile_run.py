@@ -7,6 +7,7 @@ counter pass measures both.
 
     rocprofv3 --pmc FETCH_SIZE -d OUT -o pmc --output-format csv -- \
         python -m threadcoarsenedstencilgenerator_amd.profile_run [--dtype f64|f32c] [--steps 3]
+        (--workload box2d_r1_temporal: the temporal 2D Box r=1 kernel on 16384^2 instead)
 
 With --ceiling it instead times the HBM stream probes (copy, read-only,
 write-only; plain and nontemporal; one pass and grid-stride launches) over
@@ -48,6 +49,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--dtype", default="f64", choices=("f64", "f32c"))
     ap.add_argument("--ceiling", action="store_true")
+    ap.add_argument("--workload", default="star3d_r4_512", choices=("star3d_r4_512", "box2d_r1_temporal"))
     args = ap.parse_args()
     if args.ceiling:
         return ceiling()
@@ -64,19 +66,25 @@ def main():
     torch.cuda.synchronize()
     print("calibration copy: bytes read per launch", n * 8, flush=True)
     del a, b
-    best = tuned.best("star3d_r4_512", spec.F32C if args.dtype == "f32c" else spec.F64)
-    d, block = best["desc"], best["block"]
-    k = rt.Kernel(d)
-    g = k.geometry((520, 520, 520))
-    k.set_mask(rt.mask_seeded(2, d.n))
+    if args.workload == "box2d_r1_temporal":
+        best = tuned.BOX2D_R1_TEMPORAL
+        d, block = best["desc"], best["block"]
+        k = rt.Kernel(d)
+        g = k.geometry(tuned.BOX2D_R1_GRID)
+    else:
+        best = tuned.best("star3d_r4_512", spec.F32C if args.dtype == "f32c" else spec.F64)
+        d, block = best["desc"], best["block"]
+        k = rt.Kernel(d)
+        g = k.geometry((520, 520, 520))
+        k.set_mask(rt.mask_seeded(2, d.n))
     grids = rt.Grids(g, k.dtype)
     for i in range(2):
         k.init(g, grids.ptr(i), rt.INIT_NORMAL, 1)
     k.run(g, block, rt.grid_blocks(g, block), grids.ptr(0), grids.ptr(1), args.steps)
     torch.cuda.synchronize()
     es = 4 if d.dtype == spec.F32C else 8
-    print(f"{d.key()} block {tuple(block)}: interior points {g.interior_points}, "
-          f"algorithmic bytes per launch {2 * es * g.interior_points}", flush=True)
+    print(f"{d.key()} block {tuple(block)}: interior points {g.interior_points}, steps per launch "
+          f"{k.steps_per_launch}, algorithmic bytes per launch {2 * es * g.interior_points}", flush=True)
 
 
 if __name__ == "__main__":

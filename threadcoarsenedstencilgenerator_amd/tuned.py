@@ -48,7 +48,11 @@ BOX2D_R1_GRID = (16384 + 2, 16384 + 2)
 # 128x16-point tiles with 512 threads was fastest, 670 GLup/s against 303 for the best one-step shape)
 BOX2D_R1_TEMPORAL = dict(desc=manifest.cfg1_desc("plain").replace(
     variant=TEMPORAL, tsteps=4, xc=2, yc=2, max_threads=512, remainder=True), block=(64, 8, 1))
-BOX2D_R1_ONE_STEP = dict(desc=manifest.cfg1_desc("dppw").replace(xc=2, yc=4, max_threads=256), block=(256, 1, 1))
+# (one-step: STREAM 2x4 with x-neighbours by loads, 324 GLup/s; the segment-DPP form ran at 317.
+# The wave-aligned DPP form, 301, returned wrong points at 16-lane row boundaries in 2 of 3 runs
+# of 8 steps on this grid, where the other two forms were exact in every run.)
+BOX2D_R1_ONE_STEP = dict(desc=manifest.cfg1_desc("stream").replace(xc=2, yc=4, max_threads=256),
+                         block=(256, 1, 1))
 
 
 def best(workload: str, dtype: int = F64):

@@ -920,43 +920,54 @@ class _Emitter:
         W = 2 * R + 1
         win_dx = list(range(-R, R + 1)) if box else [0]
         sn = lambda dx, k: f"u_{_n(dx)}_{k}"
-        w("  for (u32 i = m + tx; i < TCSG_LW - m; i += 64u) {")
-        w("    const TV* col = cur + i;  // signed offsets from here: the window reaches r columns left")
-        w("    const long long gx = x0 + i;")
-        w("    if (ja >= je) break;")
-        for t in range(2 * R):
-            for dx in win_dx:
-                w(f"    TV {sn(dx, t)} = TCSG_LDS(col, (int)(ja * TCSG_LW) + ({(t - R) * LW + dx}));")
-        for dx in win_dx:
-            w(f"    TV {sn(dx, 2 * R)};")
-        w(f"    for (u32 j0 = ja; j0 < je; j0 += {W}u) {{")
-        o = (0, 0, 0)
-        for q in range(W):
-            ind = "      "
-            w(f"{ind}if (j0 + {q}u < je) {{")
-            w(f"{ind}  const u32 j = j0 + {q}u;")
-            slot = lambda dy: (q + dy + R) % W
-            for dx in win_dx:
-                w(f"{ind}  {sn(dx, slot(R))} = TCSG_LDS(col, (int)(j * TCSG_LW) + ({R * LW + dx}));")
-            val = {}
-            for dy in range(-R, R + 1):
+        def walk(checked):
+            w("  for (u32 i = m + tx; i < TCSG_LW - m; i += 64u) {")
+            w("    const TV* col = cur + i;  // signed offsets from here: the window reaches r columns left")
+            w("    const long long gx = x0 + i;")
+            w("    if (ja >= je) break;")
+            for t in range(2 * R):
                 for dx in win_dx:
-                    val[(dx, dy)] = sn(dx, slot(dy))
-            if not box:
-                for dx in range(-R, R + 1):
-                    if dx:
-                        w(f"{ind}  const TV e_{_n(dx)} = TCSG_LDS(col, (int)(j * TCSG_LW) + ({dx}));")
-                        val[(dx, 0)] = f"e_{_n(dx)}"
-            w(f"{ind}  TV res = {val[(0, 0)]};")
-            w(f"{ind}  if (tcsg_interior(a, gx, y0 + j)) {{")
-            for t, (dx, dy, dz) in enumerate(self.offs):
-                mi = spec.mask_index(d, dx, dy, dz)
-                w(f"{ind}    " + self.acc_stmt(o, t, mi, val[(dx, dy)]))
-            w(f"{ind}    res = {self.final_expr(o)};")
-            w(f"{ind}  }}")
-            w(f"{ind}  TCSG_LDS(nxt, j * TCSG_LW + i) = res;")
-            w(f"{ind}}}")
-        w("    }")
+                    w(f"    TV {sn(dx, t)} = TCSG_LDS(col, (int)(ja * TCSG_LW) + ({(t - R) * LW + dx}));")
+            for dx in win_dx:
+                w(f"    TV {sn(dx, 2 * R)};")
+            w(f"    for (u32 j0 = ja; j0 < je; j0 += {W}u) {{")
+            o = (0, 0, 0)
+            for q in range(W):
+                ind = "      "
+                w(f"{ind}if (j0 + {q}u < je) {{")
+                w(f"{ind}  const u32 j = j0 + {q}u;")
+                slot = lambda dy: (q + dy + R) % W
+                for dx in win_dx:
+                    w(f"{ind}  {sn(dx, slot(R))} = TCSG_LDS(col, (int)(j * TCSG_LW) + ({R * LW + dx}));")
+                val = {}
+                for dy in range(-R, R + 1):
+                    for dx in win_dx:
+                        val[(dx, dy)] = sn(dx, slot(dy))
+                if not box:
+                    for dx in range(-R, R + 1):
+                        if dx:
+                            w(f"{ind}  const TV e_{_n(dx)} = TCSG_LDS(col, (int)(j * TCSG_LW) + ({dx}));")
+                            val[(dx, 0)] = f"e_{_n(dx)}"
+                w(f"{ind}  TV res = {val[(0, 0)]};")
+                w(f"{ind}  if ({'tcsg_interior(a, gx, y0 + j)' if checked else 'true'}) {{")
+                for t, (dx, dy, dz) in enumerate(self.offs):
+                    mi = spec.mask_index(d, dx, dy, dz)
+                    w(f"{ind}    " + self.acc_stmt(o, t, mi, val[(dx, dy)]))
+                w(f"{ind}    res = {self.final_expr(o)};")
+                w(f"{ind}  }}")
+                w(f"{ind}  TCSG_LDS(nxt, j * TCSG_LW + i) = res;")
+                w(f"{ind}}}")
+            w("    }")
+            w("  }")
+
+
+        # a sub-step region wholly inside the interior (most tiles) needs no per-point test
+        w(f"  const bool inner = x0 + m >= {R + LO} && x0 + TCSG_LW - m <= (long long)a.xdim - (long long)a.right_offset - {R} &&")
+        w(f"                     y0 + m >= {R} && y0 + TCSG_LH - m <= (long long)a.ydim - {R};")
+        w("  if (inner) {")
+        walk(False)
+        w("  } else {")
+        walk(True)
         w("  }")
         w("}")
         w("")

@@ -330,6 +330,32 @@ def bench_temporal(args):
         del grids
         k.close()
         torch.cuda.empty_cache()
+    # FP32 storage (F32C): the same number of steps from the FP32 rounding
+    # of the same grid, and its error against the FP64 temporal kernel's state
+    import numpy as np
+    c32 = tuned.BOX2D_R1_TEMPORAL_F32C
+    k64, k32 = rt.Kernel(tcfg["desc"]), rt.Kernel(c32["desc"])
+    g64, g32 = k64.geometry(tuned.BOX2D_R1_GRID), k32.geometry(tuned.BOX2D_R1_GRID)
+    G64, G32 = rt.Grids(g64, np.float64), rt.Grids(g32, np.float32)
+    for i in range(2):
+        k64.init(g64, G64.ptr(i), rt.INIT_U01, INPUT_SEED)
+        k32.init(g32, G32.ptr(i), rt.INIT_U01, INPUT_SEED)
+    steps = out["temporal"]["steps"]
+    b64, b32 = tuple(tcfg["block"]), tuple(c32["block"])
+    f64i = k64.run(g64, b64, rt.grid_blocks(g64, b64), G64.ptr(0), G64.ptr(1), steps // k64.steps_per_launch, stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    f32i = k32.run(g32, b32, rt.grid_blocks(g32, b32), G32.ptr(0), G32.ptr(1), steps // k32.steps_per_launch, stream)
+    torch.cuda.synchronize()
+    dt32 = time.perf_counter() - t0
+    mx, mb, s2, b2, _ = rt.error_f32_vs_f64(g32, G32.ptr(f32i), g64, G64.ptr(f64i), 1e-3)
+    f32c = {"glups": round(g32.interior_points * steps / dt32 / 1e9, 2), "steps": steps,
+            "steps_per_launch": k32.steps_per_launch, "tile": list(c32["desc"].tile[:2]), "block": list(b32),
+            "comp_group": c32["desc"].comp_group, "normwise_max_rel_vs_f64": float(mx / mb),
+            "normwise_rms_rel_vs_f64": float(np.sqrt(s2 / b2)), "kernel": c32["desc"].key()}
+    del G64, G32
+    k64.close()
+    k32.close()
     t, o = out["temporal"], out["one_step"]
     pts = (tuned.BOX2D_R1_GRID[0] - 2) * (tuned.BOX2D_R1_GRID[1] - 2)
     alg = 16.0 * pts / (t["kernel_ms_per_launch"] * 1e-3) / 1e9   # one read + one write per point per launch
@@ -348,7 +374,8 @@ def bench_temporal(args):
                          "note": "achieved: 16 B per point per launch (each launch advances steps_per_launch "
                                  "steps); one_step_equivalent_gbs: the HBM rate a one-step kernel would need "
                                  "for the same updates per second"},
-            "vs_one_step": round(t["glups"] / o["glups"], 3), "runs": out}
+            "vs_one_step": round(t["glups"] / o["glups"], 3), "runs": out,
+            "f32c": dict(f32c, speedup_vs_f64=round(f32c["glups"] / t["glups"], 3))}
     print(json.dumps(line), flush=True)
 
 

@@ -118,3 +118,13 @@ def test_reference_timing_protocol_counts_steps():
     assert all(x == durs[0] and x > 0 for x in durs)
     del grids
     k.close()
+
+
+def test_temporal_f32c_workload_matches_oracle():
+    """The FP32-storage temporal kernel of the 16384^2 workload: one launch
+    equals the FP32 oracle (same compensated summation) bit for bit."""
+    d = tuned.BOX2D_R1_TEMPORAL_F32C["desc"]
+    print(f"temporal workload {d.key()} size {tuned.BOX2D_R1_GRID}", flush=True)
+    got, ref = _run(d, tuned.BOX2D_R1_GRID, 1, mode=0, seed=9, halo=0.0)
+    bad = np.flatnonzero(got.view(np.uint32) != ref.view(np.uint32))
+    assert bad.size == 0, f"{bad.size} points differ, first {bad[:5]}"

@@ -51,6 +51,12 @@ BOX2D_R1_TEMPORAL = dict(desc=manifest.cfg1_desc("plain").replace(
 # (one-step: STREAM 2x4 with x-neighbours by loads, 324 GLup/s; the segment-DPP form ran at 317.
 # The wave-aligned DPP form, 301, returned wrong points at 16-lane row boundaries in 2 of 3 runs
 # of 8 steps on this grid, where the other two forms were exact in every run.)
+# FP32 storage (the F32C dtype; comp_group 9 puts all 9 terms of the Box r=1 stencil in one group,
+# a plain FP32 chain with no TwoSum): half the LDS per point, so 64x64-point tiles fit; 4 steps
+# per launch, 1169 GLup/s in the sweep
+BOX2D_R1_TEMPORAL_F32C = dict(desc=manifest.cfg1_desc("plain").replace(
+    variant=TEMPORAL, tsteps=4, xc=1, yc=8, max_threads=512, remainder=True, dtype=F32C, comp_group=9),
+    block=(64, 8, 1))
 BOX2D_R1_ONE_STEP = dict(desc=manifest.cfg1_desc("stream").replace(xc=2, yc=4, max_threads=256),
                          block=(256, 1, 1))
 
@@ -66,4 +72,4 @@ def best(workload: str, dtype: int = F64):
 def build_set():
     return [CFG2_F64["desc"], CFG2_F32C["desc"], BOX2D_R8_F64["desc"], BOX2D_R8_F32C["desc"],
             CFG2_F64["desc"].replace(remainder=True), CFG2_F32C["desc"].replace(remainder=True),
-            BOX2D_R1_TEMPORAL["desc"], BOX2D_R1_ONE_STEP["desc"]]
+            BOX2D_R1_TEMPORAL["desc"], BOX2D_R1_ONE_STEP["desc"], BOX2D_R1_TEMPORAL_F32C["desc"]]

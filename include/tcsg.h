@@ -216,7 +216,9 @@ int tcsg_batch_info(const tcsg_batch *b, uint64_t *interior_points, uint32_t *ti
  * out[0] = max |a-b|, out[1] = max |b|, out[2] = sum (a-b)^2,
  * out[3] = sum b^2, out[4] = max |a-b|/max(|b|, floor).  The layouts may
  * differ in padding (FP32 rows aligned to their own 128-byte lines) but must
- * have the same interior.  Synchronous. */
+ * have the same interior.  Synchronous (null stream).  Its 5-double reduction
+ * scratch is allocated on the first call on a device and kept; calls that
+ * overlap on one device race on it. */
 int tcsg_error_f32_vs_f64(const tcsg_geometry *ga, const float *a, const tcsg_geometry *gb, const double *b,
                           double floor, double *out);
 

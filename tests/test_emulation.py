@@ -189,16 +189,6 @@ def test_remainder_geometry_limits():
         spec.geometry(p, (2 * r + 1, 2 * r + 2, 2 * r + 4))
 
 
-def test_fp32_stream_kernels_limited_to_two_columns_per_lane():
-    """FP32-storage STREAM kernels with more than two points per lane in x are
-    refused (measured wrong values on MI355X); PLAIN FP32 kernels are not limited."""
-    with pytest.raises(ValueError):
-        StencilDesc(dimension=3, neighbourhood=STAR, orientation=ROW, coeff=VARIABLE, range=4, xc=4, yc=2, zc=64,
-                    alignment=ALIGN_WRITE, cl_size=32, dtype=F32C, variant=spec.STREAM, xext="dppw")
-    StencilDesc(dimension=2, neighbourhood=BOX, orientation=ROW, coeff=VARIABLE, range=8, xc=4, yc=4,
-                alignment=ALIGN_READ, cl_size=16, dtype=F32C, variant=spec.PLAIN)
-
-
 # the wave-aligned cases with x widened past two waves of threads, so that waves also meet inside
 # the interior and their edge lanes carry the neighbouring wave's columns (the halo is nonzero, so
 # the grid's own border counts too); each also with edge_pairs

@@ -277,6 +277,112 @@ def test_f32c_close_to_f64(group):
     assert err < 4 * 2.0 ** -24, err
 
 
+def _f32c_cancelling_grid():
+    """3x3 grid, one interior point whose 2D Star r=1 neighbourhood, in traversal
+    order (y-1, x-1, centre, x+1, y+1), holds 2^25, 1, 1, -2^25, 0.5: exact sum 2.5."""
+    g = oracle.Geometry(2, 1, ALIGN_NONE, 0, (3, 3), (1, 1))
+    a = np.zeros(g.n_points, dtype=np.float32)
+    for (x, y), v in (((1, 0), 2.0 ** 25), ((0, 1), 1.0), ((1, 1), 1.0), ((2, 1), -(2.0 ** 25)), ((1, 2), 0.5)):
+        a[x + y * g.xdim] = v
+    return g, a
+
+
+@pytest.mark.parametrize("group,expect", [(1, 2.5), (2, 0.5), (3, 0.0), (5, 0.5)])
+@pytest.mark.parametrize("coeff", [VARIABLE, JACOBI])
+def test_f32c_cancelling_sum_known_answers(group, expect, coeff):
+    """Ill-conditioned sum with exact products (weights 1): one term per group
+    (TwoSum after every term) recovers the exact 2.5, where the plain FP32 chain
+    (one group of 5) gets 0.5.  Groups of 2 and 3 lose the 1s inside a group
+    (2^25 + 1 rounds to 2^25; 1 - 2^25 ties to -2^25; -2^25 + 0.5 rounds to -2^25).
+    Dropping or overwriting the accumulated TwoSum error gives 0.5 for group 1."""
+    g, a = _f32c_cancelling_grid()
+    out = np.zeros_like(a)
+    oracle.step_f32c(D(2, STAR, ROW, coeff, 1, comp_group=group), g, np.ones(5), a, out)
+    want = np.float32(expect) if coeff == VARIABLE else np.float32(np.float32(expect) * np.float32(1.0 / 5))
+    assert out[1 + 1 * g.xdim] == want
+
+
+def _round32(x: Fraction) -> np.float32:
+    """Correctly rounded (ties to even) FP32 value of an exact rational."""
+    f = np.float32(float(x))
+    cands = [np.nextafter(f, np.float32(-np.inf)), f, np.nextafter(f, np.float32(np.inf))]
+    best = min(cands, key=lambda c: (abs(Fraction(float(c)) - x), int(np.float32(c).view(np.uint32)) & 1))
+    return np.float32(best)
+
+
+def _offsets_by_definition(dim, nb, orient, r):
+    """(offsets in traversal order, mask index of each): Box = all |d| <= r,
+    Star = centre plus axis arms; traversal lexicographic in (dz, dy, dx) for Row
+    and (dz, dx, dy) for Column; mask index = rank in (dz, dy, dx) order."""
+    rng = range(-r, r + 1)
+    pts = [(dx, dy, dz) for dz in (rng if dim == 3 else [0]) for dy in rng for dx in rng
+           if nb == BOX or sum(v != 0 for v in (dx, dy, dz)) <= 1]
+    rank = {p: k for k, p in enumerate(sorted(pts, key=lambda p: (p[2], p[1], p[0])))}
+    key = (lambda p: (p[2], p[1], p[0])) if orient == ROW else (lambda p: (p[2], p[0], p[1]))
+    order = sorted(pts, key=key)
+    return order, [rank[p] for p in order]
+
+
+def _f32c_point(vals, ws, group, jacobi, c32):
+    """Grouped FP32 sum (one correctly rounded FMA or add per term) joined by
+    Knuth's TwoSum in numpy FP32, the error terms summed and added at the end."""
+    n = len(vals)
+    s = c = q = np.float32(0)
+    for t in range(n):
+        v, w = vals[t], ws[t]
+        if t % group == 0:
+            q = v if jacobi else _round32(Fraction(float(v)) * Fraction(float(w)))
+        else:
+            q = _round32(Fraction(float(q)) + Fraction(float(v)) * (1 if jacobi else Fraction(float(w))))
+        if t % group == group - 1 or t == n - 1:
+            gi = t // group
+            if gi == 0:
+                s = q
+            else:
+                x = np.float32(s + q)
+                bp = np.float32(x - s)
+                ap = np.float32(x - bp)
+                e = np.float32(np.float32(s - ap) + np.float32(q - bp))
+                s = x
+                c = e if gi == 1 else np.float32(c + e)
+    res = np.float32(s + c) if n > group else s
+    return np.float32(res * c32) if jacobi else res
+
+
+@pytest.mark.parametrize("dim,nb,orient,coeff,r,groups", [
+    (2, STAR, ROW, VARIABLE, 2, (1, 2, 4, 9)),
+    (3, BOX, COLUMN, JACOBI, 1, (1, 5, 27)),
+    (3, STAR, ROW, VARIABLE, 4, (1, 21, 25)),
+])
+def test_f32c_matches_independent_fp32_restatement(dim, nb, orient, coeff, r, groups):
+    """oracle_step_f32c bit for bit against the numpy FP32 restatement above
+    (exactly rounded FMA from rationals, TwoSum in numpy FP32) on N(0,1) data,
+    and the compensated result (groups smaller than n) differs from the plain
+    FP32 chain (one group) at some points."""
+    size = (14, 11) if dim == 2 else ((7, 6, 6) if r == 1 else (11, 10, 10))
+    g = _grid(dim, r, size=size)
+    rng = np.random.default_rng(17)
+    a = rng.standard_normal(g.n_points).astype(np.float32)
+    order, midx = _offsets_by_definition(dim, nb, orient, r)
+    n = len(order)
+    mask = rng.uniform(-1, 1, n)
+    mask /= np.abs(mask).sum()
+    w32 = mask.astype(np.float32)
+    c32 = np.float32(1.0 / n)
+    res = {}
+    for group in groups:
+        out = np.zeros_like(a)
+        oracle.step_f32c(D(dim, nb, orient, coeff, r, comp_group=group), g, mask, a, out)
+        for (i, j, k) in _interior(g):
+            vals = [a[_idx(g, i + dx, j + dy, k + dz)] for (dx, dy, dz) in order]
+            ws = [w32[m] for m in midx]
+            want = _f32c_point(vals, ws, group, coeff == JACOBI, c32)
+            assert out[_idx(g, i, j, k)].view(np.uint32) == want.view(np.uint32), (group, i, j, k)
+        res[group] = out
+    plain = res[max(groups)]
+    assert any(np.count_nonzero(res[gr] != plain) > 0 for gr in groups if gr < n)
+
+
 def test_block_shapes_match_reference_primes():
     """Sweep order and counts against the reference's own primes.cpp
     (compiled from the reference tree into oracle/_ref when present), worked

@@ -10,7 +10,7 @@ import concurrent.futures as cf
 import os
 import subprocess
 
-from . import spec
+from . import isa_check, spec
 from .generator.emit import emit_source
 
 PKG = os.path.dirname(os.path.abspath(__file__))
@@ -75,6 +75,11 @@ def build_kernel(desc: spec.StencilDesc, force: bool = True) -> str:
             f.write(text)
         tmp = co + f".{os.getpid()}.tmp"
         _run([HIPCC, *HIP_FLAGS, "--genco", "--no-gpu-bundle-output", "-x", "hip", src, "-o", tmp])
+    try:
+        isa_check.check_code_object(tmp)
+    except RuntimeError:
+        os.remove(tmp)
+        raise
     os.replace(tmp, co)
     with open(side, "w") as f:
         f.write(digest + "\n")

@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <mutex>
 #include <vector>
 
 #include "tcsg.h"
@@ -812,25 +813,55 @@ static GridIndex grid_index(const tcsg_geometry &g)
     return GridIndex{g.size.x, g.size.y, (g.left_offset + r) + (uint64_t)g.size.x * (r + (uint64_t)g.size.y * zr)};
 }
 
-int tcsg_error_f32_vs_f64(const tcsg_geometry *ga, const float *a, const tcsg_geometry *gb, const double *b,
-                          double floor_v, double *out)
+// Reduction scratch of tcsg_error_f32_vs_f64: 5 doubles per device, allocated by the first
+// call on that device and kept until the process ends (calls that overlap on one device race).
+static std::mutex g_err_mu;
+static double *g_err_scratch[256] = {};
+
+static hipError_t error_scratch(double **out)
 {
-    if (!ga || !gb || !a || !b || !out) return TCSG_ERR_ARG;
-    if (ga->dimension != gb->dimension || ga->range != gb->range || ga->elements.x != gb->elements.x ||
-        ga->elements.y != gb->elements.y || ga->elements.z != gb->elements.z)
-        return TCSG_ERR_ARG;  // the two states must cover the same interior
-    double *d = nullptr;
-    TCSG_HIP(hipMalloc(&d, 5 * sizeof(double)));
-    hipError_t e = hipMemset(d, 0, 5 * sizeof(double));
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= 256) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_err_mu);
+    if (!g_err_scratch[dev]) {
+        e = hipMalloc(&g_err_scratch[dev], 5 * sizeof(double));
+        if (e != hipSuccess) { g_err_scratch[dev] = nullptr; return e; }
+    }
+    *out = g_err_scratch[dev];
+    return hipSuccess;
+}
+
+// Clears the 5 partial results, reduces, copies them to the host (null stream, synchronous).
+static int error_reduce(const tcsg_geometry *ga, const float *a, const tcsg_geometry *gb, const double *b,
+                        double floor_v, double *d, double *out)
+{
+    hipError_t e = hipMemsetAsync(d, 0, 5 * sizeof(double), nullptr);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(error_kernel, dim3(1024), dim3(256), 0, nullptr, a, grid_index(*ga), b, grid_index(*gb),
                            (uint64_t)ga->elements.x, (uint64_t)ga->elements.y, (uint64_t)ga->elements.z, floor_v, d);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpy(out, d, 5 * sizeof(double), hipMemcpyDeviceToHost);
-    (void)hipFree(d);
     if (e != hipSuccess) { g_last_hip = (int)e; return TCSG_ERR_HIP; }
     return TCSG_OK;
+}
+
+static bool same_interior(const tcsg_geometry *ga, const tcsg_geometry *gb)
+{
+    return ga->dimension == gb->dimension && ga->range == gb->range && ga->elements.x == gb->elements.x &&
+           ga->elements.y == gb->elements.y && ga->elements.z == gb->elements.z;
+}
+
+int tcsg_error_f32_vs_f64(const tcsg_geometry *ga, const float *a, const tcsg_geometry *gb, const double *b,
+                          double floor_v, double *out)
+{
+    if (!ga || !gb || !a || !b || !out) return TCSG_ERR_ARG;
+    if (!same_interior(ga, gb)) return TCSG_ERR_ARG;  // the two states must cover the same interior
+    double *d = nullptr;
+    TCSG_HIP(error_scratch(&d));
+    return error_reduce(ga, a, gb, b, floor_v, d, out);
 }
 
 int tcsg_convert_f64_to_f32(const double *src, float *dst, uint64_t n, void *stream)

@@ -260,7 +260,9 @@ class _Emitter:
             if cpu:
                 w("inline tcsg_emu_rsrc tcsg_rsrc(const void* p, uint32_t n) { return tcsg_make_rsrc(p, n); }")
                 w(f"inline TV tcsg_ld(const tcsg_emu_rsrc& r, const u32& off, const u32& soff) {{ return tcsg_emu_ld<{T}>(r, off, soff); }}")
-                w("inline void tcsg_st(const tcsg_emu_rsrc& r, const u32& off, const u32& soff, const TV& v) { tcsg_emu_st(r, off, soff, v); }")
+                w("inline void tcsg_st(const tcsg_emu_rsrc& r, const u32& off, const u32& soff, uint32_t imm, const TV& v) {")
+                w("  tcsg_emu_st(r, off + imm, soff, v);")
+                w("}")
                 if d.remainder and d.xc > 1:
                     # a wide store is range-checked per dword on the hardware, as its elements are here
                     es_ = 4 if self.f32 else 8
@@ -268,8 +270,9 @@ class _Emitter:
                         if n > d.xc:
                             continue
                         args = ", ".join(f"const TV& v{c}" for c in range(n))
-                        body = " ".join(f"tcsg_emu_st(r, off + {c * es_}u, soff, v{c});" for c in range(n))
-                        w(f"inline void tcsg_st{n}(const tcsg_emu_rsrc& r, const u32& off, const u32& soff, {args}) {{ {body} }}")
+                        body = " ".join(f"tcsg_emu_st(r, off + (imm + {c * es_}u), soff, v{c});" for c in range(n))
+                        w(f"inline void tcsg_st{n}(const tcsg_emu_rsrc& r, const u32& off, const u32& soff, uint32_t imm, "
+                          f"{args}) {{ {body} }}")
             else:
                 bits = 32 if self.f32 else 64
                 ity = "int" if self.f32 else "__attribute__((ext_vector_type(2))) int"
@@ -281,8 +284,27 @@ class _Emitter:
                 w(f"  return __builtin_bit_cast({T}, __builtin_amdgcn_raw_buffer_load_b{bits}(r, off, soff, {laux}));")
                 w("}")
                 aux = 2 if d.nt_store else 0
-                w(f"__device__ __forceinline__ void tcsg_st(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff, {T} v) {{")
-                w(f"  __builtin_amdgcn_raw_buffer_store_b{bits}(__builtin_bit_cast({ity}, v), r, off, soff, {aux});")
+                # a row's columns merge into one store of es*XC bytes (explicit below with
+                # remainder handling); only stores wider than 64 bits need the soffset-0 form
+                wide = (4 if self.f32 else 8) * self.XC > 8
+                if wide:
+                    w("// Stores take the row/plane offset in the per-lane offset and an soffset of 0.  On")
+                    w("// gfx950 a store of more than 64 bits (the compiler merges a row's columns into")
+                    w("// one) whose data VGPRs the next VALU instruction overwrites can store that")
+                    w("// instruction's values in some lanes, also with an SGPR soffset, which the ISA")
+                    w("// documents and the compiler exempt from the wait-state rule; with soffset 0")
+                    w("// the compiler inserts the wait states.  off + soff cannot wrap: the runtime")
+                    w("// keeps the out-of-range base plus the largest row/plane offset below 2^32.")
+                    w("// The column's constant `imm` is added last, so it becomes the instruction's")
+                    w("// immediate offset, and one add per row serves all its columns (which the")
+                    w("// compiler then merges into one wide store).")
+                    vo, so = "(off + soff) + imm", "0"
+                else:
+                    w("// Stores of at most 64 bits: row/plane offset in the SGPR soffset.")
+                    vo, so = "off + imm", "soff"
+                w(f"__device__ __forceinline__ void tcsg_st(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff, "
+                  f"uint32_t imm, {T} v) {{")
+                w(f"  __builtin_amdgcn_raw_buffer_store_b{bits}(__builtin_bit_cast({ity}, v), r, {vo}, {so}, {aux});")
                 w("}")
                 if d.remainder and d.xc > 1:
                     # one store instruction for consecutive columns of an output row
@@ -291,10 +313,11 @@ class _Emitter:
                             continue
                         args = ", ".join(f"{T} v{c}" for c in range(n))
                         vals = ", ".join(f"v{c}" for c in range(n))
-                        w(f"__device__ __forceinline__ void tcsg_st{n}(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff, {args}) {{")
+                        w(f"__device__ __forceinline__ void tcsg_st{n}(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff, "
+                          f"uint32_t imm, {args}) {{")
                         w(f"  const __attribute__((ext_vector_type({n}))) {T} v = {{{vals}}};")
                         w(f"  __builtin_amdgcn_raw_buffer_store_b{n * bits}("
-                          f"__builtin_bit_cast(__attribute__((ext_vector_type({n * bits // 32}))) int, v), r, off, soff, {aux});")
+                          f"__builtin_bit_cast(__attribute__((ext_vector_type({n * bits // 32}))) int, v), r, {vo}, {so}, {aux});")
                         w("}")
         w("")
 
@@ -760,7 +783,7 @@ class _Emitter:
                         continue
                     i, j, kk = o
                     if not xgroups:
-                        w(f"  tcsg_st(rout, sbase + {(i + XL) * es}u, {soff(j, kk)}, {self.final_expr(o)});")
+                        w(f"  tcsg_st(rout, sbase, {soff(j, kk)}, {(i + XL) * es}u, {self.final_expr(o)});")
                         continue
                     w(f"  const TV f_{i}_{j}_{kk} = {self.final_expr(o)};")
                     if idx == max(last[(c, j, kk)] for c in range(XC)):
@@ -771,12 +794,12 @@ class _Emitter:
                             n = max(m for m in (1, 2, 4) if m * es <= 16 and c0 + m <= XC)
                             part = row[c0:c0 + n]
                             if n == 1:
-                                w(f"  tcsg_st(rout, sfull + {c0 * es}u, {soff(j, kk)}, {part[0]});")
+                                w(f"  tcsg_st(rout, sfull, {soff(j, kk)}, {c0 * es}u, {part[0]});")
                             else:
-                                w(f"  tcsg_st{n}(rout, sfull + {c0 * es}u, {soff(j, kk)}, {', '.join(part)});")
+                                w(f"  tcsg_st{n}(rout, sfull, {soff(j, kk)}, {c0 * es}u, {', '.join(part)});")
                             c0 += n
                         for c in range(XC - 1):
-                            w(f"  tcsg_st(rout, spart{c}, {soff(j, kk)}, {row[c]});")
+                            w(f"  tcsg_st(rout, spart{c}, {soff(j, kk)}, 0u, {row[c]});")
 
     # ------------------------------------------------------------------ all
     # ------------------------------------------------------------- TEMPORAL

@@ -173,6 +173,23 @@ def cfg4_desc(dtype=F64, shape="dpp", batched=False, max_threads=256):
     return d.replace(batched=batched) if batched else d
 
 
+def store_hazard_cases():
+    """(name, desc, grid size incl. halo, block, steps): the STREAM kernels whose
+    16-byte stores were once followed at once by a write of their data registers,
+    at the workload sizes where they returned wrong points."""
+    box = cfg1_desc("dppw").replace(xc=2, yc=4, max_threads=256)
+    f32 = dict(shape="dppw", max_threads=256, dtype=F32C)
+    f32o = dict(prefetch=0, comp_group=21, zfirst=False, cl_size=32)
+    return [
+        ("box2d_r1_2x4_dppw_256x1", box, (16386, 16386), (256, 1, 1), 8),
+        ("box2d_r1_2x4_dppw_128x2", box, (16386, 16386), (128, 2, 1), 8),
+        ("box2d_r1_2x4_load_256x1", cfg1_desc("stream").replace(xc=2, yc=4, max_threads=256), (16386, 16386),
+         (256, 1, 1), 8),
+        ("star3d_r4_f32c_4x4x64", cfg2_desc(4, 4, 64, **f32).replace(**f32o), (520, 520, 520), (128, 2, 1), 3),
+        ("star3d_r4_f32c_4x2x128", cfg2_desc(4, 2, 128, **f32).replace(**f32o), (520, 520, 520), (128, 2, 1), 3),
+    ]
+
+
 def smoke_desc():
     """3D Star Variable r=4, coarsening 1x4x16, DPP x exchange (the smoke run's kernel)."""
     return cfg2_desc(1, 4, 16, shape="dpp", max_threads=256)
@@ -195,6 +212,7 @@ def default_build_set():
     # batched kernels with remainder handling (grids not rounded to the coarsening)
     descs += [cfg4_desc(dt, "plain", batched=True, max_threads=1024).replace(remainder=True) for dt in (F64, F32C)]
     descs += [cfg4_desc(dt, "dppw", batched=True).replace(remainder=True) for dt in (F64, F32C)]
+    descs += [c[1] for c in store_hazard_cases()]
     from . import tuned
     descs += tuned.build_set()
     return list(dict.fromkeys(descs))

@@ -114,11 +114,6 @@ class StencilDesc:
             raise ValueError("order must be 'point' or 'offset'")
         if self.order == "offset" and self.variant != PLAIN:
             raise ValueError("the offset order is a PLAIN statement order")
-        if self.variant == STREAM and self.dtype == F32C and self.xc > 2:
-            # measured on MI355X: such kernels (16-byte loads of four FP32 columns per lane)
-            # returned a varying number of wrong points from run to run, where xc <= 2 and
-            # every FP64 kernel are bit-exact; the CPU emulation of the same text is exact
-            raise ValueError("FP32-storage STREAM kernels take at most 2 points per lane in x")
         if self.edge_pairs and (self.variant != STREAM or self.xext != "dppw"):
             raise ValueError("edge_pairs is an option of the wave-aligned DPP code shape")
         if self.variant not in (PLAIN, STREAM, TEMPORAL):

@@ -15,6 +15,7 @@ write, against 8 TB/s), and the OpenMP CPU oracle timed on this host.
     python bench.py --workload batched     # 64 uneven 3D Box grids, persistent kernel + hipGraph
     python bench.py --workload box2d_r8    # 2D Box r=8 16384x8192, FP64 and FP32-compensated
     python bench.py --workload box2d_r1_temporal   # 2D Box r=1 16384^2, several steps per launch
+    python bench.py --workload star3d_r4_large     # the 512^3 kernels on 1024x1024x512 (4.5 GB buffers)
 """
 from __future__ import annotations
 
@@ -294,6 +295,55 @@ def bench_remainder(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_large(args):
+    """The 512^3 production kernel on a 1024x1024x512 interior (halo 4, Write
+    alignment: 4.53 GB per FP64 buffer, past 4 GiB), where each block addresses
+    its own window of planes with 32-bit offsets; FP64 and the FP32-storage
+    kernel (2.3 GB buffers)."""
+    import torch
+
+    from threadcoarsenedstencilgenerator_amd import runtime as rt
+    from threadcoarsenedstencilgenerator_amd import spec, tuned
+    grid = (1032, 1032, 520)
+    stream = torch.cuda.current_stream()
+    out = {}
+    for name, cfg in (("f64", tuned.best("star3d_r4_512")), ("f32c", tuned.best("star3d_r4_512", spec.F32C))):
+        k = rt.Kernel(cfg["desc"])
+        block = tuple(cfg["block"])
+        k.set_mask(rt.mask_seeded(MASK_SEED, cfg["desc"].n))
+        g = k.geometry(grid)
+        grids = rt.Grids(g, k.dtype)
+        for i in range(2):
+            k.init(g, grids.ptr(i), rt.INIT_NORMAL, INPUT_SEED)
+        blocks = rt.grid_blocks(g, block)
+        k.run(g, block, blocks, grids.ptr(0), grids.ptr(1), args.warmup, stream)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        k.run(g, block, blocks, grids.ptr(0), grids.ptr(1), args.steps, stream)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        ms = k.time_steps(g, block, blocks, grids.ptr(0), grids.ptr(1), 0, args.steps, stream) / args.steps
+        es = 4 if cfg["desc"].dtype == spec.F32C else 8
+        pts = g.interior_points
+        out[name] = {"glups": round(pts * args.steps / dt / 1e9, 2), "kernel_ms": round(ms, 4),
+                     "buffer_bytes": int(g.n_points * es),
+                     "hbm_gbs": round(2 * es * pts / (ms * 1e-3) / 1e9, 1), "kernel": cfg["desc"].key()}
+        del grids
+        k.close()
+        torch.cuda.empty_cache()
+    r = out["f64"]
+    line = {"metric": "GLup/s, FP64 3D Star r=4 1024x1024x512 (4.5 GB buffers)", "value": r["glups"],
+            "unit": "GLup/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
+            "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "3D Star Variable r=4, 1024x1024x512 interior + 4-point halo, Write alignment "
+                                   "16 points, N(0,1) values", "coarsening": list(tuned.CFG2_F64["desc"].coarsening),
+                       "block": list(tuned.CFG2_F64["block"])},
+            "roofline": {"bound": "hbm", "achieved": r["hbm_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(r["hbm_gbs"] / HBM_PEAK_GBS, 4), "traffic": None},
+            "runs": out}
+    print(json.dumps(line), flush=True)
+
+
 def bench_temporal(args):
     """2D Box Jacobi r=1 (the 1024^2 workload's stencil) on a 16384x16384
     interior, FP64, U[0,1) values: the temporal kernel (several steps per
@@ -390,7 +440,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--workload", default="star3d_r4_512", choices=("star3d_r4_512", "batched", "box2d_r8",
-                                                                    "star3d_r4_511_remainder", "box2d_r1_temporal"))
+                                                                    "star3d_r4_511_remainder", "box2d_r1_temporal",
+                                                                    "star3d_r4_large"))
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
@@ -412,6 +463,8 @@ def main():
         return bench_remainder(args)
     if args.workload == "box2d_r1_temporal":
         return bench_temporal(args)
+    if args.workload == "star3d_r4_large":
+        return bench_large(args)
     best = tuned.best("star3d_r4_512")
     desc, block = best["desc"], tuple(best["block"])
     k = rt.Kernel(desc)

@@ -47,7 +47,7 @@ def test_generator_argv_like_reference():
     # MI355X options: remainder handling (params slot 18 bit 2) and the PLAIN statement order
     r = _gen("generate3DStencil.py", "Star", "Row", "Variable", "4", "2", "4", "8", "Write", "16", "--xext=dppw",
              "--remainder")
-    assert r.returncode == 0 and "remainder=1" in r.stdout and "tcsg_st2(rout, sfull" in r.stdout
+    assert r.returncode == 0 and "remainder=1" in r.stdout and "tcsg_st2(rout0, sfull" in r.stdout
     assert re.search(r"tcsg_params\[24\] = \{(\d+, ){18}5,", r.stdout)   # nt_store | remainder << 2
     r = _gen("generate2DStencil.py", "Box", "Row", "Variable", "8", "4", "4", "--variant=plain", "--order=offset")
     assert r.returncode == 0 and "order=offset" in r.stdout
@@ -116,9 +116,13 @@ def test_geometry_check_via_c_abi():
     assert L.tcsg_geometry_check(C.byref(rt.to_cdesc(d)), C.byref(g)) == 0
     g.right_offset += 1
     assert L.tcsg_geometry_check(C.byref(rt.to_cdesc(d)), C.byref(g)) == 1
-    # a STREAM kernel's 32-bit buffer offsets cannot address a 4 GiB grid
+    # a STREAM kernel addresses one block's window of planes with 32-bit offsets: a 4.5 GB
+    # grid of 8.7 MB planes fits, planes of 17 MB (two 136-plane windows past 4 GiB) do not
     big = rt.geometry(d, (1032, 1032, 520))
-    assert L.tcsg_geometry_check(C.byref(rt.to_cdesc(d)), C.byref(big)) == 5
+    assert big.n_points * 8 > 4 << 30
+    assert L.tcsg_geometry_check(C.byref(rt.to_cdesc(d)), C.byref(big)) == 0
+    wide = rt.geometry(d, (2056, 1032, 136))
+    assert L.tcsg_geometry_check(C.byref(rt.to_cdesc(d)), C.byref(wide)) == 5
     # grids smaller than the halo are rejected
     with pytest.raises(rt.TcsgError):
         rt.geometry(d, (8, 520, 520))

@@ -66,6 +66,22 @@ def test_remainder_kernel_bitexact_511(which):
     assert bad.size == 0, f"{bad.size} points differ from the oracle"
 
 
+@pytest.mark.parametrize("which", ["f64", "f32c"])
+def test_production_kernel_bitexact_above_4gib(which):
+    """The production kernels on a 1024x1024x512 interior: FP64 buffers of 4.53 GB
+    (FP32: 2.3 GB), beyond what 32-bit offsets from the buffer start reach.  Each
+    block addresses its own window of planes; one step is bit for bit the oracle's."""
+    cfg = tuned.CFG2_F64 if which == "f64" else tuned.CFG2_F32C
+    grid = (1032, 1032, 520)
+    print(f"above-4GiB {which} {cfg['desc'].key()} grid {grid} block {cfg['block']}", flush=True)
+    got, ref = _run(cfg["desc"], tuple(cfg["block"]), 1, grid)
+    if which == "f64":
+        assert got.nbytes > 4 << 30
+    ui = np.uint32 if got.dtype == np.float32 else np.uint64
+    bad = np.flatnonzero(got.view(ui) != ref.view(ui))
+    assert bad.size == 0, f"{bad.size} points differ from the oracle"
+
+
 def test_f32c_error_vs_f64_after_20_steps():
     """FP32 storage, compensated accumulation: error of the 20-step state
     against the FP64 result of the same workload (which the test above ties

@@ -32,8 +32,6 @@ thread_local int g_last_hip = 0;
 
 constexpr uint32_t kMagic = 0x54435347u;
 constexpr uint32_t kAbi = 1u;
-// Offset given to out-of-interior lanes by STREAM kernels (the emitter's OOB_BASE).
-constexpr uint64_t kOobBase = 0xC0000000ull;
 
 // Kernel argument block; layout matches the emitted `struct tcsg_kargs`.
 struct KArgs {
@@ -41,9 +39,11 @@ struct KArgs {
     void *out;
     const void *mask;
     uint32_t xdim, ydim, zdim, right_offset;
-    uint32_t in_bytes, out_bytes, pad0, pad1;
+    uint64_t in_bytes, out_bytes;  // buffer sizes
+    uint32_t rec_limit, oob;       // STREAM: record count cap of a block's buffer window,
+                                   // base offset of lanes outside the interior
 };
-static_assert(sizeof(KArgs) == 56, "kernel argument layout");
+static_assert(sizeof(KArgs) == 64, "kernel argument layout");
 
 uint32_t stencil_size(const tcsg_desc &d)
 {
@@ -421,6 +421,37 @@ int tcsg_check_block(const tcsg_kernel *k, tcsg_dim3 b)
     return TCSG_OK;
 }
 
+// STREAM kernels address their buffers with 32-bit offsets from a base at the first
+// footprint plane (3D) or row (2D) of the launching block.  For block extent `bext` along
+// that axis:
+//   window W = (bext*zc + 2r) plane pitches (2D: (bext*yc + 2r) row pitches): the block's
+//              planes or rows;
+//   slack  S = one row pitch + (2H*xc + xc + 2r) elements: how far the edge lanes' loads
+//              reach past the window's last row;
+//   span     = (zc + 2r) plane pitches + (yc + 2r) row pitches + (2H*xc + xc + 2r) elements:
+//              the most a lane adds to its base offset.
+// Valid lanes' accesses lie below rec = W + S, the record-count cap; lanes outside the
+// interior get the base offset oob = rec + S, so their accesses (at least oob - 2H*xc
+// elements) miss the records, and oob + span < 2^32 keeps their offsets from wrapping.
+struct StreamLimits {
+    uint32_t rec, oob;
+};
+
+static int stream_limits(const tcsg_desc &d, const tcsg_geometry *g, uint32_t bext, StreamLimits *out)
+{
+    const uint64_t es = elem_size(d);
+    const uint64_t sx = (uint64_t)g->size.x * es, sxy = sx * g->size.y;
+    const uint64_t H = (d.range + d.xc - 1) / d.xc;
+    const uint64_t cols = (2 * H * d.xc + d.xc + 2ull * d.range) * es;
+    const uint64_t span = (d.zc + 2ull * d.range) * (d.dimension == 3 ? sxy : 0) + (d.yc + 2ull * d.range) * sx + cols;
+    const uint64_t window = d.dimension == 3 ? ((uint64_t)bext * d.zc + 2ull * d.range) * sxy
+                                             : ((uint64_t)bext * d.yc + 2ull * d.range) * sx;
+    const uint64_t slack = sx + cols;
+    if (window + 2 * slack + span >= (1ull << 32)) return TCSG_ERR_TOO_LARGE;
+    if (out) *out = StreamLimits{(uint32_t)(window + slack), (uint32_t)(window + 2 * slack)};
+    return TCSG_OK;
+}
+
 static int check_geom(const tcsg_desc &d, const tcsg_geometry *g, bool remainder = false)
 {
     if (g->dimension != d.dimension || g->range != d.range) return TCSG_ERR_ARG;
@@ -432,16 +463,33 @@ static int check_geom(const tcsg_desc &d, const tcsg_geometry *g, bool remainder
     if (!eq3(ref.size, g->size) || !eq3(ref.elements, g->elements) || !eq3(ref.threads, g->threads) ||
         ref.left_offset != g->left_offset || ref.right_offset != g->right_offset || ref.n_points != g->n_points)
         return TCSG_ERR_ARG;
-    if (d.variant == TCSG_STREAM) {
-        const uint64_t es = elem_size(d);
-        const uint64_t sx = (uint64_t)g->size.x * es, sxy = sx * g->size.y;
-        const uint64_t H = (d.range + d.xc - 1) / d.xc;
-        const uint64_t span = (d.zc + 2ull * d.range) * (d.dimension == 3 ? sxy : 0) +
-                              (d.yc + 2ull * d.range) * sx + (2 * H * d.xc + d.xc + 2ull * d.range) * es;
-        const uint64_t bytes = g->n_points * es;
-        if (bytes + span > kOobBase || span >= (1ull << 32) - kOobBase) return TCSG_ERR_TOO_LARGE;
-    }
+    // (the smallest block window: launches check theirs)
+    if (d.variant == TCSG_STREAM) return stream_limits(d, g, 1, nullptr);
     return TCSG_OK;
+}
+
+// Block extent along the axis a STREAM kernel's buffer window covers.
+static uint32_t window_extent(const tcsg_desc &d, tcsg_dim3 block) { return d.dimension == 3 ? block.z : block.y; }
+
+static void fill_kargs(const tcsg_desc &d, const tcsg_geometry *g, tcsg_dim3 block, const void *in, void *out,
+                       const void *mask, KArgs &a)
+{
+    a = KArgs{};
+    a.in = in;
+    a.out = out;
+    a.mask = mask;
+    a.xdim = g->size.x;
+    a.ydim = g->size.y;
+    a.zdim = g->size.z;
+    a.right_offset = g->right_offset;
+    a.in_bytes = a.out_bytes = g->n_points * elem_size(d);
+    a.rec_limit = 0xFFFFFFFFu;
+    a.oob = 0xFFFFFFFFu;
+    StreamLimits lim;
+    if (d.variant == TCSG_STREAM && stream_limits(d, g, window_extent(d, block), &lim) == TCSG_OK) {
+        a.rec_limit = lim.rec;
+        a.oob = lim.oob;
+    }
 }
 
 int tcsg_geometry_check(const tcsg_desc *desc, const tcsg_geometry *g)
@@ -489,16 +537,8 @@ int tcsg_launch_init(tcsg_kernel *k, const tcsg_geometry *g, int mode, uint64_t 
 static int launch_one(tcsg_kernel *k, const tcsg_geometry *g, tcsg_dim3 block, tcsg_dim3 blocks, const void *in,
                       void *out, hipStream_t s)
 {
-    KArgs a{};
-    a.in = in;
-    a.out = out;
-    a.mask = k->mask;
-    a.xdim = g->size.x;
-    a.ydim = g->size.y;
-    a.zdim = g->size.z;
-    a.right_offset = g->right_offset;
-    const uint64_t bytes = g->n_points * elem_size(k->desc);
-    a.in_bytes = a.out_bytes = (uint32_t)std::min<uint64_t>(bytes, 0xFFFFFFFFull);
+    KArgs a;
+    fill_kargs(k->desc, g, block, in, out, k->mask, a);
     size_t sz = sizeof(a);
     void *extra[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, &a, HIP_LAUNCH_PARAM_BUFFER_SIZE, &sz, HIP_LAUNCH_PARAM_END};
     const hipError_t e = hipModuleLaunchKernel(k->fn, blocks.x, blocks.y, blocks.z, block.x, block.y, block.z, 0, s,
@@ -518,6 +558,10 @@ static int precheck(tcsg_kernel *k, const tcsg_geometry *g, tcsg_dim3 block, tcs
     if (rc) return rc;
     rc = tcsg_check_block(k, block);
     if (rc) return rc;
+    if (k->desc.variant == TCSG_STREAM) {
+        rc = stream_limits(k->desc, g, window_extent(k->desc, block), nullptr);
+        if (rc) return rc;
+    }
     if (!blocks.x || !blocks.y || !blocks.z) return TCSG_ERR_ARG;
     if ((uint64_t)blocks.x * block.x < g->threads.x || (uint64_t)blocks.y * block.y < g->threads.y ||
         (uint64_t)blocks.z * block.z < g->threads.z)
@@ -677,9 +721,10 @@ int tcsg_batch_create(tcsg_batch **out, tcsg_kernel *k, uint32_t n, const tcsg_g
     }
     std::vector<KArgs> args[2];
     std::vector<Tile> tiles;
-    const uint64_t es = elem_size(k->desc);
     for (uint32_t i = 0; i < n; ++i) {
         rc = check_geom(k->desc, &geoms[i], k->remainder);
+        if (!rc && k->desc.variant == TCSG_STREAM)
+            rc = stream_limits(k->desc, &geoms[i], window_extent(k->desc, block), nullptr);
         if (!rc && (!bufs0[i] || !bufs1[i] || bufs0[i] == bufs1[i])) rc = TCSG_ERR_ARG;
         if (rc) { delete b; return rc; }
         const tcsg_dim3 nb = tcsg_grid_blocks(&geoms[i], block);
@@ -689,15 +734,8 @@ int tcsg_batch_create(tcsg_batch **out, tcsg_kernel *k, uint32_t n, const tcsg_g
         b->b1.push_back(bufs1[i]);
         b->points += (uint64_t)geoms[i].elements.x * geoms[i].elements.y * geoms[i].elements.z;
         for (int p = 0; p < 2; ++p) {
-            KArgs a{};
-            a.in = p ? bufs1[i] : bufs0[i];
-            a.out = p ? bufs0[i] : bufs1[i];
-            a.mask = k->mask;
-            a.xdim = geoms[i].size.x;
-            a.ydim = geoms[i].size.y;
-            a.zdim = geoms[i].size.z;
-            a.right_offset = geoms[i].right_offset;
-            a.in_bytes = a.out_bytes = (uint32_t)(geoms[i].n_points * es);
+            KArgs a;
+            fill_kargs(k->desc, &geoms[i], block, p ? bufs1[i] : bufs0[i], p ? bufs0[i] : bufs1[i], k->mask, a);
             args[p].push_back(a);
         }
         for (uint32_t z = 0; z < nb.z; ++z)

@@ -180,6 +180,18 @@ extern "C" {
 extern unsigned long tcsg_emu_bad_access;
 }
 
+// A value the hardware holds in scalar registers (wave-uniform): lane 0's value; a lane
+// that disagrees is counted in tcsg_emu_bad_access.
+template <class T>
+T tcsg_uni(const Lane<T> &v)
+{
+    for (int i = 1; i < tcsg_emu::W; ++i)
+        if (v.v[i] != v.v[0]) ++tcsg_emu_bad_access;
+    return v.v[0];
+}
+template <class T, class = std::enable_if_t<std::is_arithmetic<T>::value>>
+T tcsg_uni(T v) { return v; }
+
 template <class T>
 Lane<T> tcsg_emu_ld(const tcsg_emu_rsrc &r, const u32 &off, const u32 &soff)
 {

@@ -52,12 +52,6 @@ from .. import spec
 from ..spec import (ALIGN_NONE, ALIGN_WRITE, BOX, F32C, F64, PLAIN, ROW,
                     STREAM, TEMPORAL, VARIABLE, StencilDesc)
 
-# Byte offset given to lanes outside the interior.  The runtime only launches a
-# STREAM kernel when buffer bytes <= OOB_BASE - (largest negative footprint
-# delta) and OOB_BASE + (largest positive delta) < 2**32, so every access of
-# such a lane is past the buffer's record count and every access of a valid
-# lane is inside it.
-OOB_BASE = 0xC0000000
 
 DPP_WAVE_SHL1 = 0x130  # lane i receives lane i+1
 DPP_WAVE_SHR1 = 0x138  # lane i receives lane i-1
@@ -190,7 +184,9 @@ class _Emitter:
         w(f"  {T}* out;")
         w(f"  const {T}* mask;")
         w("  uint32_t xdim, ydim, zdim, right_offset;")
-        w("  uint32_t in_bytes, out_bytes, pad0, pad1;")
+        w("  uint64_t in_bytes, out_bytes;  // buffer sizes")
+        w("  uint32_t rec_limit, oob;       // STREAM: record count cap of a block's buffer window,")
+        w("                                 // base offset of lanes outside the interior")
         w("};")
         blob = ", ".join(str(v) for v in d.params_blob())
         w(f'extern "C" __device__ const uint32_t tcsg_params[24] = {{{blob}}};')
@@ -202,6 +198,7 @@ class _Emitter:
         if not cpu:
             w("template <class V> __device__ __forceinline__ V tcsg_sel(bool c, V a, V b) { return c ? a : b; }")
             w("__device__ __forceinline__ uint32_t tcsg_min(uint32_t a, uint32_t b) { return a < b ? a : b; }")
+            w("template <class V> __device__ __forceinline__ V tcsg_uni(V v) { return v; }  // wave-uniform value")
             w("__device__ __forceinline__ double tcsg_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }")
             w("__device__ __forceinline__ float tcsg_fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }")
         if self.f32:
@@ -553,17 +550,33 @@ class _Emitter:
             w(f"  const u32 z0 = gz * {self.ZC}u;            // first plane of the footprint")
         w(f"  const u32 sx = xdim * {es}u;            // row pitch in bytes")
         w(f"  const u32 sxy = xdim * ydim * {es}u;     // plane pitch in bytes")
-        w(f"  const u32 real = (z0 * ydim + y0) * xdim * {es}u + (x0 - {XL}u) * {es}u;")
-        w(f"  // offsets past the buffer: loads read 0, stores are dropped by the range check")
+        # the buffer resources start at the block's first footprint plane (3D) or row (2D), so
+        # 32-bit offsets reach any buffer size: the runtime only checks that one block's window
+        # (its planes or rows plus the footprint) stays below the out-of-range base
+        if self.three:
+            zcl = f", zdim - {2 * R + self.ZC}u" if d.remainder and self.ZC > 1 else ""
+            w(f"  const u32 wb = {'tcsg_min(' if zcl else '('}bzi * blockDim.z * {self.ZC}u{zcl});"
+              f"  // the block's first footprint plane")
+            w("  const uint64_t wbyte = (uint64_t)tcsg_uni(wb) * tcsg_uni(sxy);")
+            w(f"  const u32 real = ((z0 - wb) * ydim + y0) * xdim * {es}u + (x0 - {XL}u) * {es}u;")
+        else:
+            ycl = f", ydim - {2 * R + self.YC}u" if d.remainder and self.YC > 1 else ""
+            w(f"  const u32 wb = {'tcsg_min(' if ycl else '('}byi * blockDim.y * {self.YC}u{ycl});"
+              f"  // the block's first footprint row")
+            w("  const uint64_t wbyte = (uint64_t)tcsg_uni(wb) * tcsg_uni(sx);")
+            w(f"  const u32 real = (y0 - wb) * xdim * {es}u + (x0 - {XL}u) * {es}u;")
+        w("  // lanes outside the interior take base offset `oob`, past the window's record count:")
+        w("  // their loads read 0 and their stores are dropped by the range check")
+        w("  const u32 oob = a.oob;")
         if dpp:
             w("  // Lanes past the interior's x end still load (valid lanes take halo columns from")
             w("  // them through DPP).  Buffer range checks are per dword on the full offset")
             w("  // (per-lane + SGPR + immediate), so anything such a lane reads past the buffer's")
             w("  // end returns 0 and the in-range part of a straddling access is exact.")
-            w(f"  const u32 base = tcsg_sel(inrow, real, {OOB_BASE:#x}u);")
+            w(f"  const u32 base = tcsg_sel(inrow, real, oob);")
         else:
-            w(f"  const u32 base = tcsg_sel(valid, real, {OOB_BASE:#x}u);")
-        w(f"  const u32 sbase = tcsg_sel(valid, real, {OOB_BASE:#x}u);")
+            w(f"  const u32 base = tcsg_sel(valid, real, oob);")
+        w(f"  const u32 sbase = tcsg_sel(valid, real, oob);")
         # x positions stay lane-contiguous for the neighbour exchange, so with
         # remainder handling the last thread in x keeps its place and drops the
         # columns past the interior's end instead: every thread stores an output
@@ -573,12 +586,23 @@ class _Emitter:
         xgroups = d.remainder and XC > 1
         if xgroups:
             w(f"  const u32 nvx = ex - {R}u - x0;  // interior columns from x0 on")
-            w(f"  const u32 sfull = tcsg_sel(nvx >= {XC}u, sbase + {XL * es}u, {OOB_BASE:#x}u);")
+            w(f"  const u32 sfull = tcsg_sel(nvx >= {XC}u, sbase + {XL * es}u, oob);")
             for i in range(XC - 1):
                 w(f"  const u32 spart{i} = tcsg_sel(nvx > {i}u && nvx < {XC}u, sbase + {(i + XL) * es}u, "
-                  f"{OOB_BASE:#x}u);")
-        w("  const auto rin = tcsg_rsrc(a.in, a.in_bytes);")
-        w("  const auto rout = tcsg_rsrc(a.out, a.out_bytes);")
+                  f"oob);")
+        w("  const uint64_t nin = a.in_bytes - wbyte, nout = a.out_bytes - wbyte;")
+        w("  const auto rin = tcsg_rsrc((const char*)a.in + wbyte, (uint32_t)(nin < a.rec_limit ? nin : a.rec_limit));")
+        wide_st = es * XC > 8
+        if wide_st:
+            # stores wider than 64 bits take soffset 0 (see tcsg_st): one resource per output
+            # row (its row offset in the base) and the plane offset added to the per-lane
+            # offset once per plane, shared by the plane's row stores
+            w("  const uint32_t recout = (uint32_t)(nout < a.rec_limit ? nout : a.rec_limit);")
+            for j in range(self.YC):
+                w(f"  const auto rout{j} = tcsg_rsrc((const char*)a.out + (wbyte + {j + R}u * (uint64_t)tcsg_uni(sx)), "
+                  f"recout - {j + R}u * tcsg_uni(sx));")
+        else:
+            w("  const auto rout = tcsg_rsrc((const char*)a.out + wbyte, (uint32_t)(nout < a.rec_limit ? nout : a.rec_limit));")
         self.weights()
         if dpp:
             w("  const u32 lid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);")
@@ -605,12 +629,12 @@ class _Emitter:
             w("  // Extra columns of the segment-edge lanes.  Left ones are addressed only where")
             w("  // the column index stays non-negative, so no per-lane offset wraps below zero.")
             if merged_left:
-                w(f"  const u32 eoffL = tcsg_sel(edgeL, base - {H * XC * es}u, {OOB_BASE:#x}u);")
+                w(f"  const u32 eoffL = tcsg_sel(edgeL, base - {H * XC * es}u, oob);")
             else:
                 for ip in range(XC):
                     k = H * XC - ip
-                    w(f"  const u32 eoffL{ip} = tcsg_sel(edgeL && x0 >= {k}u, base - {k * es}u, {OOB_BASE:#x}u);")
-            w(f"  const u32 eoffR = tcsg_sel(edgeR, base + {H * XC * es}u, {OOB_BASE:#x}u);")
+                    w(f"  const u32 eoffL{ip} = tcsg_sel(edgeL && x0 >= {k}u, base - {k * es}u, oob);")
+            w(f"  const u32 eoffR = tcsg_sel(edgeR, base + {H * XC * es}u, oob);")
         # wave-aligned rows: the left edge lanes (lane < H) and the right ones (lane > 63 - H)
         # never coincide, so one load, with each side's offset in its own lanes, fetches both
         # sides' extra columns into one register that both lane-shift chains read
@@ -649,6 +673,12 @@ class _Emitter:
         def soff(py, pz):
             """Uniform row/plane byte offset of thread-relative row (py, pz)."""
             return f"({py + R}u * sx + {pz + ZR}u * sxy)"
+
+        def st_rs(py, pz):
+            """(buffer resource, uniform offset) of a store to output row (py, pz)."""
+            if wide_st:
+                return f"rout{py}", f"{pz + ZR}u * sxy"
+            return "rout", soff(py, pz)
 
         def vname(px, py, pz):
             return f"v_{_n(px)}_{_n(py)}_{_n(pz)}"
@@ -783,23 +813,25 @@ class _Emitter:
                         continue
                     i, j, kk = o
                     if not xgroups:
-                        w(f"  tcsg_st(rout, sbase, {soff(j, kk)}, {(i + XL) * es}u, {self.final_expr(o)});")
+                        rs, so = st_rs(j, kk)
+                        w(f"  tcsg_st({rs}, sbase, {so}, {(i + XL) * es}u, {self.final_expr(o)});")
                         continue
                     w(f"  const TV f_{i}_{j}_{kk} = {self.final_expr(o)};")
                     if idx == max(last[(c, j, kk)] for c in range(XC)):
                         row = [f"f_{c}_{j}_{kk}" for c in range(XC)]
+                        rs, so = st_rs(j, kk)
                         # wide stores of 2 or 4 elements, at most 16 bytes each
                         c0 = 0
                         while c0 < XC:
                             n = max(m for m in (1, 2, 4) if m * es <= 16 and c0 + m <= XC)
                             part = row[c0:c0 + n]
                             if n == 1:
-                                w(f"  tcsg_st(rout, sfull, {soff(j, kk)}, {c0 * es}u, {part[0]});")
+                                w(f"  tcsg_st({rs}, sfull, {so}, {c0 * es}u, {part[0]});")
                             else:
-                                w(f"  tcsg_st{n}(rout, sfull, {soff(j, kk)}, {c0 * es}u, {', '.join(part)});")
+                                w(f"  tcsg_st{n}({rs}, sfull, {so}, {c0 * es}u, {', '.join(part)});")
                             c0 += n
                         for c in range(XC - 1):
-                            w(f"  tcsg_st(rout, spart{c}, {soff(j, kk)}, 0u, {row[c]});")
+                            w(f"  tcsg_st({rs}, spart{c}, {so}, 0u, {row[c]});")
 
     # ------------------------------------------------------------------ all
     # ------------------------------------------------------------- TEMPORAL
@@ -862,7 +894,9 @@ class _Emitter:
         w(f"  {T}* out;")
         w(f"  const {T}* mask;")
         w("  uint32_t xdim, ydim, zdim, right_offset;")
-        w("  uint32_t in_bytes, out_bytes, pad0, pad1;")
+        w("  uint64_t in_bytes, out_bytes;  // buffer sizes")
+        w("  uint32_t rec_limit, oob;       // STREAM: record count cap of a block's buffer window,")
+        w("                                 // base offset of lanes outside the interior")
         w("};")
         blob = ", ".join(str(v) for v in d.params_blob())
         w(f'extern "C" {"" if cpu else "__device__ "}const uint32_t tcsg_params[24] = {{{blob}}};')
@@ -1122,18 +1156,29 @@ class _Emitter:
         w(f"  a.zdim = {'(uint32_t)zdim' if three else '1u'};")
         w(f"  a.right_offset = {'rightOffset' if aligned else '0u'};")
         w(f"  const uint64_t bytes = (uint64_t)xdim * ydim{' * zdim' if three else ''} * 8u;")
-        w("  a.in_bytes = a.out_bytes = (uint32_t)(bytes < 0xFFFFFFFFull ? bytes : 0xFFFFFFFFull);")
-        w("  a.pad0 = a.pad1 = 0u;")
+        w("  a.in_bytes = a.out_bytes = bytes;")
+        w("  a.rec_limit = a.oob = 0xFFFFFFFFu;")
         if fallback:
             H = self.halo_lanes()
-            w("  // this kernel's body needs buffers within its 32-bit addressing (lanes outside the")
-            w("  // interior are sent 3 GiB past the buffer start) and, for its x-neighbour exchange,")
-            w("  // the block shapes the runtime's tcsg_check_block accepts; the reference's code")
-            w("  // shape, summing in the same order, takes the launches that do not fit")
+            w("  // this kernel's body addresses one block's window of planes (rows in 2D) with 32-bit")
+            w("  // offsets (the limits the runtime's launches compute: record cap rec = window + slack,")
+            w("  // base offset oob = rec + slack of lanes outside the interior, no wrap below 2^32)")
+            w("  // and, for its x-neighbour exchange, needs the block shapes the runtime's")
+            w("  // tcsg_check_block accepts; the reference's code shape, summing in the same order,")
+            w("  // takes the launches that do not fit")
             w("  const uint64_t sx = (uint64_t)xdim * 8u, sxy = sx * ydim;")
-            w(f"  const uint64_t span = {d.zc + 2 * R}ull * {'sxy' if three else '0ull'} + {d.yc + 2 * R}ull * sx + "
-              f"{(2 * H * d.xc + d.xc + 2 * R) * 8}ull;")
-            w(f"  bool fast = bytes + span <= {OOB_BASE:#x}ull;")
+            w(f"  const uint64_t cols = {(2 * H * d.xc + d.xc + 2 * R) * 8}ull;")
+            w(f"  const uint64_t span = {d.zc + 2 * R}ull * {'sxy' if three else '0ull'} + {d.yc + 2 * R}ull * sx + cols;")
+            if three:
+                w(f"  const uint64_t window = ((uint64_t)blockDim.z * {d.zc}u + {2 * R}u) * sxy;")
+            else:
+                w(f"  const uint64_t window = ((uint64_t)blockDim.y * {d.yc}u + {2 * R}u) * sx;")
+            w("  const uint64_t slack = sx + cols;")
+            w("  bool fast = window + 2u * slack + span < (1ull << 32);")
+            w("  if (fast) {")
+            w("    a.rec_limit = (uint32_t)(window + slack);")
+            w("    a.oob = (uint32_t)(window + 2u * slack);")
+            w("  }")
             if d.xext == "dppw":
                 w("  fast = fast && blockDim.x % 64u == 0u;")
             elif d.xext == "dpp":

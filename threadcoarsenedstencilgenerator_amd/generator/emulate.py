@@ -24,7 +24,7 @@ EMU_DIR = os.path.join(tempfile.gettempdir(), f"tcsg_emu_{os.getuid()}")
 class _KArgs(C.Structure):
     _fields_ = [("in_", C.c_void_p), ("out", C.c_void_p), ("mask", C.c_void_p),
                 ("xdim", C.c_uint32), ("ydim", C.c_uint32), ("zdim", C.c_uint32), ("right_offset", C.c_uint32),
-                ("in_bytes", C.c_uint32), ("out_bytes", C.c_uint32), ("pad0", C.c_uint32), ("pad1", C.c_uint32)]
+                ("in_bytes", C.c_uint64), ("out_bytes", C.c_uint64), ("rec_limit", C.c_uint32), ("oob", C.c_uint32)]
 
 
 def build(desc: spec.StencilDesc) -> str:
@@ -47,6 +47,25 @@ def build(desc: spec.StencilDesc) -> str:
     return so
 
 
+def stream_limits(desc: spec.StencilDesc, geom, bext: int):
+    """(record-count cap, out-of-range base offset) of a STREAM block's buffer window
+    for block extent `bext` along z (3D) or y (2D), as the runtime sets them
+    (0xFFFFFFFF for other shapes)."""
+    if desc.variant != spec.STREAM:
+        return 0xFFFFFFFF, 0xFFFFFFFF
+    es = 4 if desc.dtype == F32C else 8
+    sx = geom.xdim * es
+    sxy = sx * geom.ydim
+    r = desc.range
+    h = (r + desc.xc - 1) // desc.xc
+    cols = (2 * h * desc.xc + desc.xc + 2 * r) * es
+    span = (desc.zc + 2 * r) * (sxy if desc.dimension == 3 else 0) + (desc.yc + 2 * r) * sx + cols
+    window = (bext * desc.zc + 2 * r) * sxy if desc.dimension == 3 else (bext * desc.yc + 2 * r) * sx
+    slack = sx + cols
+    assert window + 2 * slack + span < 1 << 32
+    return window + slack, window + 2 * slack
+
+
 def run(desc: spec.StencilDesc, geom, mask, a: np.ndarray, b: np.ndarray, steps: int, block, blocks):
     """`steps` ping-pong steps of the emulated kernel on host arrays a/b
     (geometry as from spec.geometry).  Returns (final array, bad accesses)."""
@@ -58,7 +77,8 @@ def run(desc: spec.StencilDesc, geom, mask, a: np.ndarray, b: np.ndarray, steps:
     nbytes = a.nbytes
     for s in range(steps):
         k = _KArgs(bufs[s & 1].ctypes.data, bufs[(s + 1) & 1].ctypes.data, m.ctypes.data,
-                   geom.xdim, geom.ydim, geom.zdim, geom.ro, nbytes, nbytes, 0, 0)
+                   geom.xdim, geom.ydim, geom.zdim, geom.ro, nbytes, nbytes,
+                   *stream_limits(desc, geom, block[2] if desc.dimension == 3 else block[1]))
         rc = L.tcsg_emulate(C.byref(k), *blocks, *block)
         if rc != 0:
             raise ValueError("block shape must be whole waves")
@@ -89,7 +109,8 @@ def run_batched(desc: spec.StencilDesc, geoms, arrays0, arrays1, mask, steps: in
         for i, g in enumerate(geoms):
             src, dst = bufs[s & 1][i], bufs[(s + 1) & 1][i]
             ka[i] = _KArgs(src.ctypes.data, dst.ctypes.data, m.ctypes.data, g.xdim, g.ydim, g.zdim, g.ro,
-                           src.nbytes, dst.nbytes, 0, 0)
+                           src.nbytes, dst.nbytes,
+                           *stream_limits(desc, g, block[2] if desc.dimension == 3 else block[1]))
         if L.tcsg_emulate_batched(C.cast(ka, C.c_void_p), tiles.ctypes.data, len(tiles), queue.ctypes.data, nwg,
                                   *block) != 0:
             raise ValueError("block shape must be whole waves")

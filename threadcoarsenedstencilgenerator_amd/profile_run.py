@@ -7,7 +7,8 @@ counter pass measures both.
 
     rocprofv3 --pmc FETCH_SIZE -d OUT -o pmc --output-format csv -- \
         python -m threadcoarsenedstencilgenerator_amd.profile_run [--dtype f64|f32c] [--steps 3]
-        (--workload box2d_r1_temporal: the temporal 2D Box r=1 kernel on 16384^2 instead)
+        (--workload box2d_r1_temporal: the temporal 2D Box r=1 kernel on 16384^2 instead;
+         --workload all --meta FILE: every measured workload in one pass, see all_workloads)
 
 With --ceiling it instead times the HBM stream probes (copy, read-only,
 write-only; plain and nontemporal; one pass and grid-stride launches) over
@@ -44,15 +45,99 @@ def ceiling(reps: int = 20):
     print(json.dumps({"hbm_ceiling_tbs": best, "peak_tbs": 8.0}), flush=True)
 
 
+def _setup(k, g, mode, seed):
+    from . import runtime as rt
+    grids = rt.Grids(g, k.dtype)
+    for i in range(2):
+        k.init(g, grids.ptr(i), mode, seed)
+    return grids
+
+
+def all_workloads(steps: int, meta: str):
+    """One pass over every measured workload (the counter report's dispatches):
+    the 2^30-byte calibration copy, then `steps` launches each of the 512^3 FP64
+    and FP32C production kernels, the 2D Box r=8 16384x8192 FP64 kernel, the
+    batched persistent kernel over the 64 uneven grids (no graph) and the 2D
+    temporal kernel on 16384^2.  Writes `meta`: per workload the kernel symbol,
+    launch grid and workgroup (threads), interior points, time steps per launch,
+    flops per update (reference count) and algorithmic bytes per update."""
+    import json
+
+    import torch
+
+    from . import manifest, spec, tuned
+    from . import runtime as rt
+    torch.cuda.set_device(0)
+    n = 1 << 27
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
+    out = [dict(name="copy_2^30B", symbol="__amd_rocclr_copyBuffer", bytes_per_launch=1 << 30)]
+
+    def record(name, k, d, g, block, kind="single"):
+        blocks = rt.grid_blocks(g, block)
+        es = 4 if d.dtype == spec.F32C else 8
+        out.append(dict(name=name, kernel=d.key(), symbol="tcsg_update", kind=kind,
+                        grid_threads=[blocks[i] * block[i] for i in range(3)], workgroup=list(block),
+                        interior_points=g.interior_points, steps_per_launch=k.steps_per_launch,
+                        flops_per_update=d.flops_per_update, bytes_per_update=2 * es))
+        return blocks
+
+    for name, cfg, grid, mode in (("star3d_r4_512_f64", tuned.CFG2_F64, (520, 520, 520), rt.INIT_NORMAL),
+                                  ("star3d_r4_512_f32c", tuned.CFG2_F32C, (520, 520, 520), rt.INIT_NORMAL),
+                                  ("box2d_r8_16384x8192_f64", tuned.BOX2D_R8_F64, tuned.BOX2D_R8_GRID, rt.INIT_UPM1),
+                                  ("box2d_r1_16384_temporal_f64", tuned.BOX2D_R1_TEMPORAL, tuned.BOX2D_R1_GRID,
+                                   rt.INIT_U01)):
+        d, block = cfg["desc"], tuple(cfg["block"])
+        k = rt.Kernel(d)
+        if d.coeff == spec.VARIABLE:
+            k.set_mask(rt.mask_seeded(2, d.n))
+        g = k.geometry(grid)
+        grids = _setup(k, g, mode, 1)
+        blocks = record(name, k, d, g, block)
+        k.run(g, block, blocks, grids.ptr(0), grids.ptr(1), steps)
+        torch.cuda.synchronize()
+        print(f"{name}: {d.key()} block {block} grid {blocks}", flush=True)
+        del grids
+        k.close()
+        torch.cuda.empty_cache()
+    # batched: 64 uneven 3D Box Jacobi r=2 grids, one persistent launch per step
+    d = manifest.cfg4_desc(shape="plain", batched=True, max_threads=1024)
+    k = rt.Kernel(d)
+    sizes = rt.batch_sizes(64, 1, 32, 256, d.coarsening, d.range)
+    geoms = [k.geometry(sz) for sz in sizes]
+    gr = [_setup(k, g, rt.INIT_U01, 100 + i) for i, g in enumerate(geoms)]
+    block = (32, 8, 2)
+    bt = rt.Batch(k, geoms, [x.ptr(0) for x in gr], [x.ptr(1) for x in gr], block)
+    pts, tiles, nwg = bt.info()
+    out.append(dict(name="batched_64_box3d_r2_f64", kernel=d.key(), symbol="tcsg_update_batched", kind="batched",
+                    grid_threads=[nwg * block[0] * block[1] * block[2], 1, 1], workgroup=list(block),
+                    interior_points=pts, steps_per_launch=1, flops_per_update=d.flops_per_update,
+                    bytes_per_update=16, tiles=tiles))
+    bt.run(steps, graph=False)
+    torch.cuda.synchronize()
+    print(f"batched: {d.key()} {tiles} tiles, {nwg} workgroups", flush=True)
+    bt.close()
+    with open(meta, "w") as f:
+        json.dump(out, f, indent=1)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--dtype", default="f64", choices=("f64", "f32c"))
     ap.add_argument("--ceiling", action="store_true")
-    ap.add_argument("--workload", default="star3d_r4_512", choices=("star3d_r4_512", "box2d_r1_temporal"))
+    ap.add_argument("--workload", default="star3d_r4_512", choices=("star3d_r4_512", "box2d_r1_temporal", "all"))
+    ap.add_argument("--meta", default="profile_workloads.json", help="--workload all: where to write the "
+                    "dispatch table")
     args = ap.parse_args()
     if args.ceiling:
         return ceiling()
+    if args.workload == "all":
+        return all_workloads(args.steps, args.meta)
     import torch
 
     from . import runtime as rt

@@ -235,6 +235,13 @@ int tcsg_convert_f64_to_f32(const double *src, float *dst, uint64_t n, void *str
 int tcsg_hbm_probe(int mode, int nontemporal, const void *src, void *dst, uint64_t bytes, uint32_t workgroups,
                    int reps, void *stream, float *ms_per_rep);
 
+/* The same probe with `unroll` 16-byte elements per lane per iteration (1, 2,
+ * 4, 8 or 16: that many loads in flight per lane before the stores) and
+ * `block` threads per workgroup (256, 512 or 1024); tcsg_hbm_probe is unroll 4,
+ * block 256.  No reference counterpart. */
+int tcsg_hbm_probe_ex(int mode, int nontemporal, uint32_t unroll, uint32_t block, const void *src, void *dst,
+                      uint64_t bytes, uint32_t workgroups, int reps, void *stream, float *ms_per_rep);
+
 /* ---- cost-model network of the learned block-shape tuner (no reference
  * counterpart: the reference measures every block shape, hip/benchmark.cpp:
  * 221-263; the tuner ranks them with this network and measures the top few).

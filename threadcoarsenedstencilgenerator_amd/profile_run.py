@@ -11,7 +11,8 @@ counter pass measures both.
          --workload all --meta FILE: every measured workload in one pass, see all_workloads)
 
 With --ceiling it instead times the HBM stream probes (copy, read-only,
-write-only; plain and nontemporal; one pass and grid-stride launches) over
+write-only; plain and nontemporal; 1-16 loads of 16 bytes in flight per lane,
+256-1024 threads per workgroup; one pass and grid-stride launches) over
 2 GiB buffers (past every cache level, about twice the 512^3 workload's grid)
 and prints one JSON line per probe: the practical bandwidth ceiling the
 stencil kernels are compared with.
@@ -35,12 +36,16 @@ def ceiling(reps: int = 20):
     for mode in ("copy", "read", "write"):
         moved = 2 * nbytes if mode == "copy" else nbytes
         for nt in (False, True):
-            for wg in (0, 256 * 4, 256 * 8, 256 * 16, 256 * 32):
-                ms = rt.hbm_probe(mode, a.data_ptr(), b.data_ptr(), nbytes, nt, wg, reps)
-                line = dict(probe=mode, nontemporal=nt, workgroups=wg or "one pass", bytes_per_launch=moved,
-                            ms=round(ms, 4), tbs=round(moved / ms / 1e9, 3))
-                out.append(line)
-                print(json.dumps(line), flush=True)
+            for unroll, block in ((1, 256), (2, 256), (4, 256), (8, 256), (16, 256), (4, 512), (4, 1024),
+                                  (8, 512), (16, 1024)):
+                for wg in (0, 256 * 4, 256 * 8, 256 * 16, 256 * 32):
+                    ms = rt.hbm_probe(mode, a.data_ptr(), b.data_ptr(), nbytes, nt, wg, reps, unroll=unroll,
+                                      block=block)
+                    line = dict(probe=mode, nontemporal=nt, unroll=unroll, block=block,
+                                workgroups=wg or "one pass", bytes_per_launch=moved, ms=round(ms, 4),
+                                tbs=round(moved / ms / 1e9, 3))
+                    out.append(line)
+                    print(json.dumps(line), flush=True)
     best = {m: max(x["tbs"] for x in out if x["probe"] == m) for m in ("copy", "read", "write")}
     print(json.dumps({"hbm_ceiling_tbs": best, "peak_tbs": 8.0}), flush=True)
 

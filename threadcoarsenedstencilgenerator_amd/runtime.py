@@ -105,6 +105,7 @@ def _lib():
                                             C.POINTER(C.c_double)]
         L.tcsg_convert_f64_to_f32.argtypes = [p, p, u64, p]
         L.tcsg_hbm_probe.argtypes = [i, i, p, p, u64, u32, i, p, C.POINTER(C.c_float)]
+        L.tcsg_hbm_probe_ex.argtypes = [i, i, u32, u32, p, p, u64, u32, i, p, C.POINTER(C.c_float)]
         L.tcsg_mlp_num_params.argtypes = [u32, u32]
         L.tcsg_mlp_num_params.restype = C.c_size_t
         L.tcsg_mlp_create.argtypes = [C.POINTER(p), u32, u32, u32, u64]
@@ -286,11 +287,12 @@ def convert_f64_to_f32(src_ptr: int, dst_ptr: int, n: int, stream=None):
 
 
 def hbm_probe(mode: str, src: int, dst: int, nbytes: int, nontemporal: bool = False, workgroups: int = 0,
-              reps: int = 10, stream=None) -> float:
-    """Milliseconds per launch of the copy / read / write bandwidth probe."""
+              reps: int = 10, stream=None, unroll: int = 4, block: int = 256) -> float:
+    """Milliseconds per launch of the copy / read / write bandwidth probe
+    (`unroll` 16-byte elements per lane in flight, `block` threads per workgroup)."""
     ms = C.c_float()
-    _check(_lib().tcsg_hbm_probe({"copy": 0, "read": 1, "write": 2}[mode], int(nontemporal), src, dst, nbytes,
-                                 workgroups, reps, _stream(stream), C.byref(ms)), "hbm_probe")
+    _check(_lib().tcsg_hbm_probe_ex({"copy": 0, "read": 1, "write": 2}[mode], int(nontemporal), unroll, block, src,
+                                    dst, nbytes, workgroups, reps, _stream(stream), C.byref(ms)), "hbm_probe")
     return ms.value
 
 

@@ -167,6 +167,20 @@ def report(meta, passes, trace):
             r["ta_busy_frac"] = g("TA_TA_BUSY_sum") / 256 / (g("GRBM_GUI_ACTIVE") / 8)
         if g("GRBM_GUI_ACTIVE") and g("TD_TD_BUSY_sum") is not None:
             r["td_busy_frac"] = g("TD_TD_BUSY_sum") / 256 / (g("GRBM_GUI_ACTIVE") / 8)
+        cyc = g("GRBM_GUI_ACTIVE") and g("GRBM_GUI_ACTIVE") / 8   # cycles of one XCD's clock domain
+        if cyc and g("TCP_PENDING_STALL_CYCLES_sum") is not None:
+            # per-CU fractions of the dispatch's cycles (256 CUs)
+            r["l1_waiting_on_l2_frac"] = g("TCP_PENDING_STALL_CYCLES_sum") / 256 / cyc
+        if cyc and g("TA_DATA_STALLED_BY_TC_CYCLES_sum") is not None:
+            r["ta_data_stalled_frac"] = g("TA_DATA_STALLED_BY_TC_CYCLES_sum") / 256 / cyc
+        if g("TCP_TCC_READ_REQ_sum"):
+            r["l2_read_requests_per_update"] = g("TCP_TCC_READ_REQ_sum") / upd
+            if g("TCP_TCC_READ_REQ_LATENCY_sum"):
+                lat = g("TCP_TCC_READ_REQ_LATENCY_sum") / g("TCP_TCC_READ_REQ_sum")
+                r["l2_read_latency_cycles"] = lat
+                if cyc:
+                    # Little's law: L2 reads in flight per CU
+                    r["l2_reads_in_flight_per_cu"] = g("TCP_TCC_READ_REQ_sum") / 256 / cyc * lat
         rows.append(r)
     return dict(fetch_scale=fscale, write_scale=wscale, rows=rows)
 
@@ -199,7 +213,11 @@ def markdown(rep) -> str:
              ("lds_bytes_per_update", "LDS B/update", ".1f"), ("l2_hit_rate", "L2 hit", ".3f"),
              ("wait_inst_frac", "wave time waiting", ".3f"), ("valu_active_frac_of_wave_time", "VALU active", ".3f"),
              ("vmem_active_frac_of_wave_time", "VMEM active", ".3f"), ("ta_busy_frac", "TA busy", ".3f"),
-             ("td_busy_frac", "TD busy", ".3f")]
+             ("td_busy_frac", "TD busy", ".3f"), ("ta_data_stalled_frac", "TA stalled on L1", ".3f"),
+             ("l1_waiting_on_l2_frac", "L1 waiting on L2", ".3f"),
+             ("l2_read_requests_per_update", "L2 reads/update", ".3f"),
+             ("l2_read_latency_cycles", "L2 read latency [cyc]", ".0f"),
+             ("l2_reads_in_flight_per_cu", "L2 reads in flight/CU", ".0f")]
     out += ["", "| " + " | ".join(h for _, h, _ in cols2) + " |", "|" + "---|" * len(cols2)]
     for r in rep["rows"]:
         out.append("| " + " | ".join(r["workload"] if k == "workload" else _f(r.get(k), sp)

@@ -222,6 +222,18 @@ int tcsg_batch_info(const tcsg_batch *b, uint64_t *interior_points, uint32_t *ti
 int tcsg_error_f32_vs_f64(const tcsg_geometry *ga, const float *a, const tcsg_geometry *gb, const double *b,
                           double floor, double *out);
 
+/* The same error reduction without allocating anything, for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops, harnesses
+ * that forbid hipMalloc).  No counterpart in the reference.  Asynchronous on
+ * `stream`; the 5 results (as above) go to the device array `out`.  Two passes with
+ * a fixed summation order (deterministic, unlike tcsg_error_f32_vs_f64's atomics):
+ * per-workgroup partial results in the library's static device scratch of 16,384
+ * rows of 5 doubles (640 KiB on each GPU it runs on, held until the process ends),
+ * so interiors of at most 2^30 points; larger ones return TCSG_ERR_TOO_LARGE.
+ * Every call on a GPU shares that GPU's scratch: calls that overlap on one GPU race. */
+int tcsg_error_f32_vs_f64_noalloc(const tcsg_geometry *ga, const float *a, const tcsg_geometry *gb,
+                                  const double *b, double floor_v, double *out, void *stream);
+
 /* FP64 -> FP32 copy of a whole grid (round to nearest).  Asynchronous. */
 int tcsg_convert_f64_to_f32(const double *src, float *dst, uint64_t n, void *stream);
 

@@ -902,6 +902,122 @@ int tcsg_error_f32_vs_f64(const tcsg_geometry *ga, const float *a, const tcsg_ge
     return error_reduce(ga, a, gb, b, floor_v, d, out);
 }
 
+// ---- the error reduction without allocation (no counterpart in the reference) ----
+// Partial results of the two-pass reduction: one row of 5 per workgroup of the first
+// pass, in a file-scope device array (kErrBlocks rows, 640 KiB on each GPU the library
+// runs on, held until the process ends).  Each first-pass workgroup reduces a contiguous
+// range of at most kErrPointsPerBlock interior points, so interiors up to
+// kErrBlocks * kErrPointsPerBlock = 2^30 points fit; larger ones are refused.
+constexpr uint32_t kErrBlocks = 16384;
+constexpr uint32_t kErrThreads = 256;
+constexpr uint64_t kErrPointsPerBlock = 65536;
+static __device__ __align__(256) double scratch[kErrBlocks * 5];
+
+// first pass: workgroup w reduces points [w*chunk, min((w+1)*chunk, n)) in a fixed order
+// (each thread its strided points in turn, then a fixed tree over the threads) into row w
+__global__ void __launch_bounds__(kErrThreads) error_partial_kernel(const float *__restrict__ a, GridIndex ga,
+                                                                    const double *__restrict__ b, GridIndex gb,
+                                                                    uint64_t nx, uint64_t ny, uint64_t nz,
+                                                                    uint64_t chunk, double floor_v, double *part)
+{
+    const uint64_t n = nx * ny * nz;
+    const uint64_t p0 = blockIdx.x * chunk, p1 = p0 + chunk < n ? p0 + chunk : n;
+    double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (uint64_t t = p0 + threadIdx.x; t < p1; t += kErrThreads) {
+        const uint64_t i = t % nx, jk = t / nx, j = jk % ny, k = jk / ny;
+        const double bv = b[gb.at(i, j, k)];
+        const double d = fabs((double)a[ga.at(i, j, k)] - bv);
+        v[0] = fmax(v[0], d);
+        v[1] = fmax(v[1], fabs(bv));
+        v[2] += d * d;
+        v[3] += bv * bv;
+        v[4] = fmax(v[4], d / fmax(fabs(bv), floor_v));
+    }
+    __shared__ double red[5][kErrThreads];
+    for (int c = 0; c < 5; ++c) red[c][threadIdx.x] = v[c];
+    __syncthreads();
+    for (uint32_t h = kErrThreads / 2; h > 0; h >>= 1) {
+        if (threadIdx.x < h) {
+            for (int c = 0; c < 5; ++c) {
+                const double o = red[c][threadIdx.x + h];
+                red[c][threadIdx.x] = (c == 2 || c == 3) ? red[c][threadIdx.x] + o : fmax(red[c][threadIdx.x], o);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 5) part[5 * (uint64_t)blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// second pass: one workgroup folds the rows [0, rows) in a fixed order into out[5]
+__global__ void __launch_bounds__(kErrThreads) error_final_kernel(const double *__restrict__ part, uint32_t rows,
+                                                                  double *__restrict__ out)
+{
+    double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (uint32_t w = threadIdx.x; w < rows; w += kErrThreads) {
+        for (int c = 0; c < 5; ++c) {
+            const double o = part[5 * (uint64_t)w + c];
+            v[c] = (c == 2 || c == 3) ? v[c] + o : fmax(v[c], o);
+        }
+    }
+    __shared__ double red[5][kErrThreads];
+    for (int c = 0; c < 5; ++c) red[c][threadIdx.x] = v[c];
+    __syncthreads();
+    for (uint32_t h = kErrThreads / 2; h > 0; h >>= 1) {
+        if (threadIdx.x < h) {
+            for (int c = 0; c < 5; ++c) {
+                const double o = red[c][threadIdx.x + h];
+                red[c][threadIdx.x] = (c == 2 || c == 3) ? red[c][threadIdx.x] + o : fmax(red[c][threadIdx.x], o);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 5) out[threadIdx.x] = red[threadIdx.x][0];
+}
+
+// Device address of `scratch` on each device, looked up on the first call on that device
+// (with it current); host code uses these addresses, never the array's name.
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[256] = {};
+
+static hipError_t scratch_address(double **out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= 256) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = (double *)ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
+
+int tcsg_error_f32_vs_f64_noalloc(const tcsg_geometry *ga, const float *a, const tcsg_geometry *gb,
+                                  const double *b, double floor_v, double *out, void *stream)
+{
+    if (!ga || !gb || !a || !b || !out) return TCSG_ERR_ARG;
+    if (!same_interior(ga, gb)) return TCSG_ERR_ARG;  // the two states must cover the same interior
+    const uint64_t n = (uint64_t)ga->elements.x * ga->elements.y * ga->elements.z;
+    if (n == 0) return TCSG_ERR_ARG;
+    if (n > (uint64_t)kErrBlocks * kErrPointsPerBlock) return TCSG_ERR_TOO_LARGE;  // beyond the scratch rows
+    double *part = nullptr;
+    TCSG_HIP(scratch_address(&part));
+    const uint64_t chunk = std::max<uint64_t>(kErrThreads, (n + kErrBlocks - 1) / kErrBlocks);
+    const uint32_t rows = (uint32_t)((n + chunk - 1) / chunk);
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(error_partial_kernel, dim3(rows), dim3(kErrThreads), 0, s, a, grid_index(*ga), b,
+                       grid_index(*gb), (uint64_t)ga->elements.x, (uint64_t)ga->elements.y,
+                       (uint64_t)ga->elements.z, chunk, floor_v, part);
+    TCSG_HIP(hipGetLastError());
+    hipLaunchKernelGGL(error_final_kernel, dim3(1), dim3(kErrThreads), 0, s, (const double *)part, rows, out);
+    TCSG_HIP(hipGetLastError());
+    return TCSG_OK;
+}
+
 int tcsg_convert_f64_to_f32(const double *src, float *dst, uint64_t n, void *stream)
 {
     if (!src || !dst) return TCSG_ERR_ARG;

@@ -103,6 +103,8 @@ def _lib():
         L.tcsg_batch_info.argtypes = [p, C.POINTER(u64), C.POINTER(u32), C.POINTER(u32)]
         L.tcsg_error_f32_vs_f64.argtypes = [C.POINTER(CGeometry), p, C.POINTER(CGeometry), p, C.c_double,
                                             C.POINTER(C.c_double)]
+        L.tcsg_error_f32_vs_f64_noalloc.argtypes = [C.POINTER(CGeometry), p, C.POINTER(CGeometry), p, C.c_double,
+                                                    p, p]
         L.tcsg_convert_f64_to_f32.argtypes = [p, p, u64, p]
         L.tcsg_hbm_probe.argtypes = [i, i, p, p, u64, u32, i, p, C.POINTER(C.c_float)]
         L.tcsg_hbm_probe_ex.argtypes = [i, i, u32, u32, p, p, u64, u32, i, p, C.POINTER(C.c_float)]
@@ -280,6 +282,15 @@ def error_f32_vs_f64(ga: CGeometry, a_ptr: int, gb: CGeometry, b_ptr: int, floor
     out = (C.c_double * 5)()
     _check(_lib().tcsg_error_f32_vs_f64(C.byref(ga), a_ptr, C.byref(gb), b_ptr, floor, out), "error_f32_vs_f64")
     return tuple(out)
+
+
+def error_f32_vs_f64_noalloc(ga: CGeometry, a_ptr: int, gb: CGeometry, b_ptr: int, out_ptr: int,
+                             floor: float = 1e-3, stream=None):
+    """The same reduction without allocating: deterministic two passes through the
+    library's static scratch, asynchronous on `stream`, the 5 results into the
+    device array at out_ptr."""
+    _check(_lib().tcsg_error_f32_vs_f64_noalloc(C.byref(ga), a_ptr, C.byref(gb), b_ptr, floor, out_ptr,
+                                                _stream(stream)), "error_f32_vs_f64_noalloc")
 
 
 def convert_f64_to_f32(src_ptr: int, dst_ptr: int, n: int, stream=None):

@@ -16,6 +16,7 @@ write, against 8 TB/s), and the OpenMP CPU oracle timed on this host.
     python bench.py --workload box2d_r8    # 2D Box r=8 16384x8192, FP64 and FP32-compensated
     python bench.py --workload box2d_r1_temporal   # 2D Box r=1 16384^2, several steps per launch
     python bench.py --workload star3d_r4_large     # the 512^3 kernels on 1024x1024x512 (4.5 GB buffers)
+    python bench.py --workload star3d_r1_temporal  # 3D Star r=1 512^3, 2 steps per launch (z-march, LDS rings)
 """
 from __future__ import annotations
 
@@ -432,6 +433,94 @@ def bench_temporal(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_temporal3d(args):
+    """3D Star Variable r=1 (7 weights) on a 512^3 interior, FP64, N(0,1) values:
+    the 3D temporal kernel (2 steps per launch, z-march with plane rings in LDS)
+    against the fastest one-step kernel on the same grid, the same number of
+    steps each, and the OpenMP oracle on this host."""
+    import torch
+
+    from threadcoarsenedstencilgenerator_amd import runtime as rt
+    from threadcoarsenedstencilgenerator_amd import tuned
+    tcfg, ocfg = tuned.STAR3D_R1_TEMPORAL, tuned.STAR3D_R1_ONE_STEP
+    stream = torch.cuda.current_stream()
+    mask = rt.mask_seeded(MASK_SEED, tcfg["desc"].n)
+    out = {}
+    for name, cfg in (("temporal", tcfg), ("one_step", ocfg)):
+        k = rt.Kernel(cfg["desc"])
+        k.set_mask(mask)
+        block = tuple(cfg["block"])
+        K = k.steps_per_launch
+        launches = max(1, args.steps // K)
+        g = k.geometry(tuned.STAR3D_R1_GRID)
+        grids = rt.Grids(g, k.dtype)
+        for i in range(2):
+            k.init(g, grids.ptr(i), rt.INIT_NORMAL, INPUT_SEED)
+        blocks = rt.grid_blocks(g, block)
+        k.run(g, block, blocks, grids.ptr(0), grids.ptr(1), max(1, args.warmup // K), stream)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        k.run(g, block, blocks, grids.ptr(0), grids.ptr(1), launches, stream)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        ms = k.time_steps(g, block, blocks, grids.ptr(0), grids.ptr(1), 0, launches, stream) / launches
+        pts = g.interior_points
+        out[name] = {"glups": round(pts * K * launches / dt / 1e9, 2), "steps": K * launches,
+                     "steps_per_launch": K, "kernel_ms_per_launch": round(ms, 4),
+                     "kernel_glups": round(pts * K / (ms * 1e-3) / 1e9, 2), "block": list(block),
+                     "kernel": cfg["desc"].key()}
+        del grids
+        k.close()
+        torch.cuda.empty_cache()
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline_grid(tcfg["desc"], tuned.STAR3D_R1_GRID, min(args.cpu_seconds, 10.0),
+                                "3D Star r=1 Variable 512^3")
+    t, o = out["temporal"], out["one_step"]
+    pts = 512 ** 3
+    alg = 16.0 * pts / (t["kernel_ms_per_launch"] * 1e-3) / 1e9   # one read + one write per point per launch
+    line = {"metric": "GLup/s, FP64 3D Star r=1 512^3, temporal blocking", "value": t["glups"],
+            "unit": "GLup/s", "n_gpus": 1, "steps": t["steps"], "warmup": args.warmup, "higher_is_better": True,
+            "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "3D Star Variable r=1 (7 weights, sum|w|=1), 512^3 interior + 1-point halo, "
+                                   "FP64, alignment None, N(0,1) values",
+                       "steps_per_launch": t["steps_per_launch"],
+                       "tile": [64 * tcfg["desc"].xc, tcfg["desc"].max_threads // 64 * tcfg["desc"].yc,
+                                tcfg["desc"].zc], "block": t["block"]},
+            "roofline": {"bound": "hbm", "achieved": round(alg, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(alg / HBM_PEAK_GBS, 4), "traffic": None,
+                         "one_step_equivalent_gbs": round(16.0 * t["kernel_glups"], 1),
+                         "note": "achieved: 16 B per point per launch (each launch advances steps_per_launch "
+                                 "steps); one_step_equivalent_gbs: the HBM rate a one-step kernel would need "
+                                 "for the same updates per second"},
+            "vs_one_step": round(t["glups"] / o["glups"], 3), "runs": out, "cpu_baseline": cpu}
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline_grid(desc, grid, budget_s: float, what: str):
+    """The OpenMP C oracle on this host for `desc` on `grid` (seeded N(0,1)
+    interior): as many whole steps as fit in about budget_s."""
+    import numpy as np
+
+    import oracle
+    g = oracle.Geometry(desc.dimension, desc.range, desc.alignment, desc.cl_size, grid, (1, 1, 1))
+    a = np.empty(g.n_points)
+    b = np.empty(g.n_points)
+    oracle.init_seeded(g, 2, INPUT_SEED, a)
+    b[:] = a
+    mask = oracle.mask_seeded(MASK_SEED, desc.n)
+    t0 = time.perf_counter()
+    oracle.step_f64(desc, g, mask, a, b)
+    one = time.perf_counter() - t0
+    steps = max(1, min(400, int(budget_s / max(one, 1e-3)) - 1))
+    t0 = time.perf_counter()
+    oracle.run_f64(desc, g, mask, b, a, steps)
+    dt = time.perf_counter() - t0
+    return {"value": round(g.interior_points * steps / dt / 1e9, 4), "unit": "GLup/s",
+            "cores": oracle.num_threads(), "kind": "port",
+            "sample": f"{steps} FP64 time steps of the same {what} grid ({dt:.1f} s), OpenMP C oracle"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -441,7 +530,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--workload", default="star3d_r4_512", choices=("star3d_r4_512", "batched", "box2d_r8",
                                                                     "star3d_r4_511_remainder", "box2d_r1_temporal",
-                                                                    "star3d_r4_large"))
+                                                                    "star3d_r4_large", "star3d_r1_temporal"))
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
@@ -465,6 +554,8 @@ def main():
         return bench_temporal(args)
     if args.workload == "star3d_r4_large":
         return bench_large(args)
+    if args.workload == "star3d_r1_temporal":
+        return bench_temporal3d(args)
     best = tuned.best("star3d_r4_512")
     desc, block = best["desc"], tuple(best["block"])
     k = rt.Kernel(desc)

@@ -250,8 +250,11 @@ def test_emulated_temporal_kernel_matches_oracle(case):
 
 def test_temporal_descriptor_limits():
     T = spec.TEMPORAL
-    with pytest.raises(ValueError, match="2D"):
-        StencilDesc(dimension=3, variant=T, tsteps=2)
+    # 3D: a ring of 2r+1 planes per level s < tsteps, level s covering the tile plus (tsteps-s) r per side
+    d3 = StencilDesc(dimension=3, variant=T, range=1, yc=2, zc=128, tsteps=2, max_threads=512)
+    assert d3.tile == (64, 16, 68, 20) and d3.lds_bytes == 3 * (68 * 20 + 66 * 18) * 8
+    with pytest.raises(ValueError, match="LDS"):
+        StencilDesc(dimension=3, variant=T, range=2, yc=4, zc=16, tsteps=3, max_threads=512)
     with pytest.raises(ValueError, match="LDS"):
         StencilDesc(dimension=2, variant=T, range=4, xc=4, yc=8, tsteps=4, max_threads=256)
     with pytest.raises(ValueError, match="temporal variant"):
@@ -261,3 +264,32 @@ def test_temporal_descriptor_limits():
     d = StencilDesc(dimension=2, neighbourhood=BOX, range=1, yc=8, variant=T, tsteps=4, max_threads=256)
     assert d.tile == (64, 32, 72, 40) and d.lds_bytes == 2 * 72 * 40 * 8
     assert d.params_blob()[13] == 2 and d.params_blob()[18] == 4 << 8
+
+
+TEMPORAL3D_CASES = manifest.temporal3d_cases()
+
+
+@pytest.mark.parametrize("case", TEMPORAL3D_CASES, ids=[c[0].key() for c in TEMPORAL3D_CASES])
+def test_emulated_temporal3d_kernel_matches_oracle(case):
+    """Two launches of a 3D kernel that marches through z advancing tsteps steps
+    per launch equal 2*tsteps steps of the oracle bit for bit, halo and padding
+    included, with every LDS access inside the workgroup's rings."""
+    d, size = case
+    geo = spec.geometry(d, size)
+    og = oracle.Geometry(3, d.range, d.alignment, d.cl_size, size, (1, 1, 1) if d.remainder else d.coarsening)
+    assert (og.xdim, og.ydim, og.zdim) == (geo.xdim, geo.ydim, geo.zdim)
+    a = np.zeros(og.n_points)
+    oracle.init_seeded(og, 1, 4, a, halo=0.375)
+    mask = oracle.mask_seeded(6, d.n)
+    dt = np.float32 if d.dtype == F32C else np.float64
+    a = a.astype(dt)
+    run = oracle.run_f32c if d.dtype == F32C else oracle.run_f64
+    ref = run(d, og, mask, a.copy(), a.copy(), 2 * d.tsteps)
+    block = (64, d.max_threads // 64, 1)
+    nblk = tuple((geo.threads[i] + block[i] - 1) // block[i] for i in range(3))
+    assert nblk[0] > 1 and nblk[1] > 1 and nblk[2] > 1
+    got, bad = emulate.run(d, geo, mask.astype(dt), a.copy(), a.copy(), 2, block, nblk)
+    assert bad == 0, f"{bad} LDS accesses outside the rings"
+    ui = np.uint32 if dt == np.float32 else np.uint64
+    diff = np.flatnonzero(got.view(ui) != ref.view(ui))
+    assert diff.size == 0, f"{diff.size} points differ, first {diff[:5]}"

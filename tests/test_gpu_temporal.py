@@ -24,8 +24,9 @@ def _run(d, size, launches, mode=1, seed=4, halo=0.375):
     if d.coeff == spec.VARIABLE:
         k.set_mask(mask)
     g = k.geometry(size)
-    og = oracle.Geometry(2, d.range, d.alignment, d.cl_size, size, (1, 1, 1) if d.remainder else d.coarsening)
-    assert (og.xdim, og.ydim) == (g.size.x, g.size.y)
+    og = oracle.Geometry(d.dimension, d.range, d.alignment, d.cl_size, size,
+                         (1, 1, 1) if d.remainder else d.coarsening)
+    assert (og.xdim, og.ydim, og.zdim) == (g.size.x, g.size.y, g.size.z)
     a = np.zeros(og.n_points)
     oracle.init_seeded(og, mode, seed, a, halo=halo)
     dt = np.float32 if d.dtype == spec.F32C else np.float64
@@ -48,6 +49,18 @@ def _run(d, size, launches, mode=1, seed=4, halo=0.375):
 def test_temporal_kernel_matches_oracle(case):
     d, size = case
     print(f"temporal {d.key()} size {size}", flush=True)
+    got, ref = _run(d, size, 3)
+    ui = np.uint32 if d.dtype == spec.F32C else np.uint64
+    bad = np.flatnonzero(got.view(ui) != ref.view(ui))
+    assert bad.size == 0, f"{bad.size} points differ, first {bad[:5]}"
+
+
+@pytest.mark.parametrize("case", manifest.temporal3d_cases(), ids=[c[0].key() for c in manifest.temporal3d_cases()])
+def test_temporal3d_kernel_matches_oracle(case):
+    """3D kernels marching through z, tsteps steps per launch: 3 launches equal
+    3*tsteps oracle steps bit for bit on grids of several tiles per axis."""
+    d, size = case
+    print(f"temporal 3D {d.key()} size {size}", flush=True)
     got, ref = _run(d, size, 3)
     ui = np.uint32 if d.dtype == spec.F32C else np.uint64
     bad = np.flatnonzero(got.view(ui) != ref.view(ui))
@@ -128,3 +141,40 @@ def test_temporal_f32c_workload_matches_oracle():
     got, ref = _run(d, tuned.BOX2D_R1_GRID, 1, mode=0, seed=9, halo=0.0)
     bad = np.flatnonzero(got.view(np.uint32) != ref.view(np.uint32))
     assert bad.size == 0, f"{bad.size} points differ, first {bad[:5]}"
+
+
+def test_temporal3d_workload_matches_oracle_and_one_step_kernel():
+    """The tuned 3D temporal kernel on the 512^3 3D Star r=1 Variable grid: two
+    launches (4 steps) equal the oracle and 4 launches of the one-step STREAM kernel
+    of the same stencil bit for bit."""
+    import torch
+
+    import oracle
+    from threadcoarsenedstencilgenerator_amd import runtime as rt
+    cfg, one = tuned.STAR3D_R1_TEMPORAL, tuned.STAR3D_R1_ONE_STEP
+    size = tuned.STAR3D_R1_GRID
+    mask = rt.mask_seeded(2, cfg["desc"].n)
+    finals = []
+    a0 = None
+    for c, launches in ((cfg, 2), (one, 4)):
+        d, b = c["desc"], tuple(c["block"])
+        print(f"temporal 3D workload {d.key()} size {size} block {b} launches {launches}", flush=True)
+        k = rt.Kernel(d)
+        k.set_mask(mask)
+        g = k.geometry(size)
+        grids = rt.Grids(g, k.dtype)
+        for i in range(2):
+            k.init(g, grids.ptr(i), rt.INIT_NORMAL, 3)
+        torch.cuda.synchronize()
+        if a0 is None:
+            a0 = grids.host(0)
+        fin = k.run(g, b, rt.grid_blocks(g, b), grids.ptr(0), grids.ptr(1), launches)
+        torch.cuda.synchronize()
+        finals.append(grids.host(fin))
+        del grids
+        k.close()
+        torch.cuda.empty_cache()
+    og = oracle.Geometry(3, 1, 0, 0, size, (1, 1, 1))
+    ref = oracle.run_f64(cfg["desc"], og, mask, a0.copy(), a0.copy(), 4)
+    assert np.array_equal(finals[0].view(np.uint64), ref.view(np.uint64))
+    assert np.array_equal(finals[1].view(np.uint64), finals[0].view(np.uint64))

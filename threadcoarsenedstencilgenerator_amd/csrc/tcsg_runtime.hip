@@ -58,7 +58,6 @@ bool desc_valid(const tcsg_desc &d)
 {
     if (d.dimension != 2 && d.dimension != 3) return false;
     if (d.neighbourhood > 1 || d.orientation > 1 || d.coeff > 1 || d.dtype > 1 || d.variant > 2) return false;
-    if (d.variant == TCSG_TEMPORAL && d.dimension != 2) return false;
     if (d.range < 1 || d.xc < 1 || d.yc < 1 || d.zc < 1) return false;
     if (d.dimension == 2 && d.zc != 1) return false;
     if (d.alignment > 2) return false;

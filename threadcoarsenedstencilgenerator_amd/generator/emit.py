@@ -835,31 +835,16 @@ class _Emitter:
 
     # ------------------------------------------------------------------ all
     # ------------------------------------------------------------- TEMPORAL
-    def source_temporal(self) -> str:
-        """2D kernel that advances tsteps time steps per launch.  Workgroup
-        (64, BY) owns an output tile of TX = 64 xc by TY = BY yc points; it
-        loads the tile plus H = tsteps*r points per side into LDS, then each
-        sub-step s updates the region s*r inside the LDS tile from the previous
-        sub-step's values (ping-pong between two LDS buffers), and the last
-        sub-step's tile goes to the output.  Points outside the interior (the
-        Dirichlet halo) keep their values; points past the allocation are read
-        as 0 and are never inputs of an interior point.  Every interior point
-        sums its terms in the reference's order with the same operations as a
-        one-step kernel, so k launches equal k*tsteps steps bit for bit."""
+    def _temporal_head(self, title: str, lds_elems: int):
+        """Includes, helpers, the argument struct, the parameter block and the LDS
+        access macro of a temporal kernel (in the CPU build every LDS access is
+        range-checked against the `lds_elems` elements of the workgroup's LDS)."""
         d = self.d
         T = self.T
         w = self.w
         cpu = self.target == "cpu"
-        R, LO, K = self.R, d.left_offset, d.tsteps
-        H = K * R
-        BY = d.max_threads // 64
-        TX, TY, LW, LH = d.tile
-        nbname = "Box" if d.neighbourhood == BOX else "Star"
         w("// Generated by threadcoarsenedstencilgenerator_amd.generator -- do not edit.")
-        w(f"// 2D {nbname} r={R} {'Variable' if d.coeff == VARIABLE else 'Jacobi'} "
-          f"{'Row' if d.orientation == ROW else 'Column'} alignment {['None', 'Read', 'Write'][d.alignment]}/"
-          f"{d.cl_size} {'f32c' if self.f32 else 'f64'} temporal: {K} steps per launch, tile {TX}x{TY}, "
-          f"block 64x{BY}" + (" (CPU emulation build)" if cpu else ""))
+        w(f"// {title}" + (" (CPU emulation build)" if cpu else ""))
         fn = "static inline" if cpu else "__device__ __forceinline__"
         if cpu:
             w("#include <cmath>")
@@ -904,18 +889,233 @@ class _Emitter:
             w(f"__attribute__((unused)) static const uint32_t tcsg_params_host[24] = {{{blob}}};")
         w("")
         if cpu:
-            w("// LDS accesses are range-checked here: one outside the two tile buffers is counted")
+            w("// LDS accesses are range-checked here: one outside the workgroup's LDS is counted")
             w("// (and reads a dummy) instead of touching other memory")
             w('extern "C" { unsigned long tcsg_emu_bad_access = 0; }')
             w("static TV* tcsg_lds_lo = nullptr;")
             w("static TV tcsg_lds_dummy;")
             w(f"static inline TV& tcsg_lds(TV* p) {{")
-            w(f"  if (p < tcsg_lds_lo || p >= tcsg_lds_lo + 2 * {LW * LH}) {{ ++tcsg_emu_bad_access; return tcsg_lds_dummy; }}")
+            w(f"  if (p < tcsg_lds_lo || p >= tcsg_lds_lo + {lds_elems}) {{ ++tcsg_emu_bad_access; return tcsg_lds_dummy; }}")
             w("  return *p;")
             w("}")
             w("#define TCSG_LDS(p, k) tcsg_lds(const_cast<TV*>(p) + (k))")
         else:
             w("#define TCSG_LDS(p, k) (p)[k]")
+
+    def source_temporal3d(self) -> str:
+        """3D kernel that advances tsteps = K time steps per launch by marching
+        through z.  Workgroup (64, BY) owns the outputs of an x-y tile of TX = 64 xc
+        by TY = BY yc points over ZC = zc consecutive planes.  Level 0 holds input
+        planes of the tile plus H = K r points per side, level s (1..K-1) the step-s
+        values of planes over the tile plus (K - s) r points per side; each level is a
+        ring of 2r+1 planes in LDS.  Iteration t loads input plane t (the next one is
+        fetched into registers while this one is used), then each level s computes
+        its plane t - s r from the 2r+1 planes of level s-1 around it; level K writes
+        the output plane.  Points outside the interior (the Dirichlet halo) keep
+        their values; points past the allocation read 0 and feed no interior point.
+        Every interior point sums its terms in the reference's order with the same
+        operations as a one-step kernel, so k launches equal k*K steps bit for bit."""
+        d = self.d
+        T = self.T
+        w = self.w
+        cpu = self.target == "cpu"
+        R, LO, K = self.R, d.left_offset, d.tsteps
+        H = K * R
+        BY = d.max_threads // 64
+        NT = 64 * BY
+        TX, TY, ZC = 64 * d.xc, BY * d.yc, d.zc
+        D = 2 * R + 1
+        NQ = ZC + 2 * H                                   # input planes per workgroup
+        dims = [(TX + 2 * (K - s) * R, TY + 2 * (K - s) * R) for s in range(K)]   # level s region
+        offs, o = [], 0
+        for (ws, hs) in dims:
+            offs.append(o)
+            o += D * ws * hs
+        total = o
+        nbname = "Box" if d.neighbourhood == BOX else "Star"
+        self._temporal_head(f"3D {nbname} r={R} {'Variable' if d.coeff == VARIABLE else 'Jacobi'} "
+                            f"{'Row' if d.orientation == ROW else 'Column'} alignment "
+                            f"{['None', 'Read', 'Write'][d.alignment]}/{d.cl_size} {'f32c' if self.f32 else 'f64'} "
+                            f"temporal: {K} steps per launch, tile {TX}x{TY}x{ZC}, block 64x{BY}, "
+                            f"LDS {total * (4 if self.f32 else 8)} bytes", total)
+        fn = "static inline" if cpu else "__device__ __forceinline__"
+        W0, H0 = dims[0]
+        NLD = (W0 * H0 + NT - 1) // NT
+        w("// interior of the padded grid")
+        w(f"{fn} bool tcsg_interior(const tcsg_kargs& a, long long gx, long long gy, long long gz) {{")
+        w(f"  return gx >= {R + LO} && gx < (long long)a.xdim - (long long)a.right_offset - {R} &&")
+        w(f"         gy >= {R} && gy < (long long)a.ydim - {R} && gz >= {R} && gz < (long long)a.zdim - {R};")
+        w("}")
+        w("// global x, y of level 0's first column and row, and z of input plane 0")
+        w(f"{fn} long long tcsg_x0(u32 bx) {{ return (long long)bx * {TX} + {R + LO - H}; }}")
+        w(f"{fn} long long tcsg_y0(u32 by) {{ return (long long)by * {TY} + {R - H}; }}")
+        w(f"{fn} long long tcsg_z0(u32 bz) {{ return (long long)bz * {ZC} + {R - H}; }}")
+        w("")
+        w(f"// the {d.n if d.coeff == VARIABLE else 0} weights, read once per workgroup before any plane is fetched")
+        w(f"struct tcsg_w {{ TV w[{d.n if d.coeff == VARIABLE else 1}]; }};")
+        w(f"{fn} void tcsg_weights(const tcsg_kargs& a, tcsg_w& W) {{")
+        if d.coeff == VARIABLE:
+            for mi in range(d.n):
+                w(f"  W.w[{mi}] = a.mask[{mi}];")
+        else:
+            w("  W.w[0] = 0;")
+        w("}")
+        w(f"// one input plane of level 0 ({W0}x{H0} points), {NLD} per thread, in registers")
+        w(f"struct tcsg_pf {{ TV v[{NLD}]; }};")
+        w(f"{fn} void tcsg_fetch(const tcsg_kargs& a, u32 bx, u32 by, u32 bz, u32 tid, u32 t, tcsg_pf& pf) {{")
+        w("  const long long x0 = tcsg_x0(bx), y0 = tcsg_y0(by), gz = tcsg_z0(bz) + (long long)t;")
+        w("  const bool zok = gz >= 0 && gz < (long long)a.zdim;")
+        for k in range(NLD):
+            w(f"  {{ const u32 p = tid + {k * NT}u;")
+            w(f"    const long long gx = x0 + (long long)(p % {W0}u), gy = y0 + (long long)(p / {W0}u);")
+            cond = f"p < {W0 * H0}u && " if (k + 1) * NT > W0 * H0 else ""
+            w(f"    const bool ok = {cond}zok && gx >= 0 && gx < (long long)a.xdim && gy >= 0 && gy < (long long)a.ydim;")
+            w(f"    pf.v[{k}] = ok ? a.in[((size_t)gz * a.ydim + (size_t)gy) * a.xdim + (size_t)gx] : ({T})0; }}")
+        w("}")
+        w(f"{fn} void tcsg_put(TV* lds, u32 tid, u32 t, const tcsg_pf& pf) {{")
+        w(f"  TV* pl = lds + {offs[0]}u + (t % {D}u) * {W0 * H0}u;")
+        for k in range(NLD):
+            st = f"TCSG_LDS(pl, tid + {k * NT}u) = pf.v[{k}];"
+            w(f"  if (tid + {k * NT}u < {W0 * H0}u) {st}" if (k + 1) * NT > W0 * H0 else f"  {st}")
+        w("}")
+        w("")
+        box = d.neighbourhood == BOX
+        for s_ in range(1, K + 1):
+            Wp, Hp = dims[s_ - 1]
+            last = s_ == K
+            Ws, Hs = (TX, TY) if last else dims[s_]
+            w(f"// level {s_}: plane q = t - {s_ * R} over {Ws}x{Hs} points from level {s_ - 1}"
+              + (" into the output grid" if last else ""))
+            w(f"{fn} void tcsg_level{s_}(const tcsg_kargs& a, const tcsg_w& W, TV* lds, u32 bx, u32 by, u32 bz, "
+              f"u32 tid, u32 t) {{")
+            w(f"  if (t < {2 * s_ * R}u || t >= {NQ}u) return;   // planes q in [{s_ * R}, {NQ - s_ * R})")
+            w(f"  const u32 q = t - {s_ * R}u;")
+            w(f"  const long long gx0 = tcsg_x0(bx) + {s_ * R}, gy0 = tcsg_y0(by) + {s_ * R}, "
+              f"gz = tcsg_z0(bz) + (long long)q;")
+            for dz in range(-R, R + 1):
+                w(f"  const TV* src{_n(dz)} = lds + {offs[s_ - 1]}u + ((q + {D + dz}u) % {D}u) * {Wp * Hp}u;")
+            if not last:
+                w(f"  TV* dst = lds + {offs[s_]}u + (q % {D}u) * {Ws * Hs}u;")
+            if d.coeff == VARIABLE:
+                for mi in range(d.n):
+                    w(f"  const TV w{mi} = W.w[{mi}];")
+            npt = (Ws * Hs + NT - 1) // NT
+
+            def points(checked):
+                # the thread's points p = tid + k*NT, k < npt, unrolled so that every point's
+                # LDS reads can be in flight together
+                for k in range(npt):
+                    guard = (k + 1) * NT > Ws * Hs
+                    w(f"    {{ const u32 p = tid + {k * NT}u;")
+                    if guard:
+                        w(f"      if (p < {Ws * Hs}u) {{")
+                    ind = "        " if guard else "      "
+                    w(f"{ind}const u32 i = p % {Ws}u, j = p / {Ws}u;")
+                    w(f"{ind}const u32 c = (j + {R}u) * {Wp}u + i + {R}u;   // the point in level {s_ - 1}")
+                    if checked:
+                        w(f"{ind}const bool in = tcsg_interior(a, gx0 + (long long)i, gy0 + (long long)j, gz);")
+                    w(f"{ind}TV res = TCSG_LDS(src0, c);")
+                    w(f"{ind}if (in) {{" if checked else f"{ind}{{")
+                    for t, (dx, dy, dz) in enumerate(self.offs):
+                        mi = spec.mask_index(d, dx, dy, dz)
+                        v = f"TCSG_LDS(src{_n(dz)}, c + ({dy * Wp + dx}))"
+                        w(f"{ind}  " + self.acc_stmt((0, 0, 0), t, mi, v))
+                    w(f"{ind}  res = {self.final_expr((0, 0, 0))};")
+                    w(f"{ind}}}")
+                    if last:
+                        st = "a.out[((size_t)gz * a.ydim + (size_t)(gy0 + j)) * a.xdim + (size_t)(gx0 + i)] = res;"
+                        w(f"{ind}if (in) {st}" if checked else f"{ind}{st}")
+                    else:
+                        w(f"{ind}TCSG_LDS(dst, p) = res;")
+                    if guard:
+                        w("      }")
+                    w("    }")
+            # a plane of the level wholly inside the interior (most of them) needs no per-point test
+            w(f"  const bool inner = gz >= {R} && gz < (long long)a.zdim - {R} && gx0 >= {R + LO} && "
+              f"gx0 + {Ws} <= (long long)a.xdim - (long long)a.right_offset - {R} &&")
+            w(f"                     gy0 >= {R} && gy0 + {Hs} <= (long long)a.ydim - {R};")
+            w("  if (inner) {")
+            points(False)
+            w("  } else {")
+            points(True)
+            w("  }")
+            w("}")
+            w("")
+        # barriers: after the plane's put, and after each level but the last (whose reads of
+        # level K-1 end before the next iteration's put and its barrier); with one step per
+        # launch the level also reads level 0, which the next put overwrites
+        syncs = [True] * (K - 1) + [K == 1]
+        if cpu:
+            w("// One launch: the workgroups one after another; within a workgroup each phase runs for")
+            w("// every thread before the next phase starts (the kernel's barriers).")
+            w('extern "C" int tcsg_emulate(const tcsg_kargs* a, uint32_t gx, uint32_t gy, uint32_t gz,')
+            w("                            uint32_t bx, uint32_t by, uint32_t bz) {")
+            w(f"  if (bx != 64u || by != {BY}u || bz != 1u) return -1;")
+            w(f"  std::vector<TV> lds({total});")
+            w("  tcsg_lds_lo = lds.data();")
+            w(f"  std::vector<tcsg_pf> pf({NT});")
+            w("  tcsg_w W;")
+            w("  tcsg_weights(*a, W);")
+            w("  for (uint32_t wz = 0; wz < gz; ++wz) for (uint32_t wy = 0; wy < gy; ++wy) for (uint32_t wx = 0; wx < gx; ++wx) {")
+            w(f"    for (u32 tid = 0; tid < {NT}u; ++tid) tcsg_fetch(*a, wx, wy, wz, tid, 0u, pf[tid]);")
+            w(f"    for (u32 t = 0; t < {NQ}u; ++t) {{")
+            w(f"      for (u32 tid = 0; tid < {NT}u; ++tid) {{")
+            w("        tcsg_put(lds.data(), tid, t, pf[tid]);")
+            w(f"        if (t + 1u < {NQ}u) tcsg_fetch(*a, wx, wy, wz, tid, t + 1u, pf[tid]);")
+            w("      }")
+            for s_ in range(1, K + 1):
+                w(f"      for (u32 tid = 0; tid < {NT}u; ++tid) tcsg_level{s_}(*a, W, lds.data(), wx, wy, wz, tid, t);")
+            w("    }")
+            w("  }")
+            w("  return 0;")
+            w("}")
+        else:
+            w(f'extern "C" __global__ void __launch_bounds__({d.max_threads}) tcsg_update(const tcsg_kargs a)')
+            w("{")
+            w(f"  __shared__ TV lds[{total}];")
+            w("  const u32 tid = threadIdx.x + 64u * threadIdx.y;")
+            w("  const u32 bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;")
+            w("  tcsg_w W;")
+            w("  tcsg_weights(a, W);")
+            w("  tcsg_pf pf;")
+            w("  tcsg_fetch(a, bx, by, bz, tid, 0u, pf);")
+            w(f"  for (u32 t = 0; t < {NQ}u; ++t) {{")
+            w("    tcsg_put(lds, tid, t, pf);")
+            w(f"    if (t + 1u < {NQ}u) tcsg_fetch(a, bx, by, bz, tid, t + 1u, pf);")
+            w("    __syncthreads();")
+            for s_ in range(1, K + 1):
+                w(f"    tcsg_level{s_}(a, W, lds, bx, by, bz, tid, t);")
+                if syncs[s_ - 1]:
+                    w("    __syncthreads();")
+            w("  }")
+            w("}")
+        return "\n".join(self.L) + "\n"
+
+    def source_temporal(self) -> str:
+        """2D kernel that advances tsteps time steps per launch.  Workgroup
+        (64, BY) owns an output tile of TX = 64 xc by TY = BY yc points; it
+        loads the tile plus H = tsteps*r points per side into LDS, then each
+        sub-step s updates the region s*r inside the LDS tile from the previous
+        sub-step's values (ping-pong between two LDS buffers), and the last
+        sub-step's tile goes to the output.  Points outside the interior (the
+        Dirichlet halo) keep their values; points past the allocation are read
+        as 0 and are never inputs of an interior point.  Every interior point
+        sums its terms in the reference's order with the same operations as a
+        one-step kernel, so k launches equal k*tsteps steps bit for bit."""
+        d = self.d
+        T = self.T
+        w = self.w
+        cpu = self.target == "cpu"
+        R, LO, K = self.R, d.left_offset, d.tsteps
+        H = K * R
+        BY = d.max_threads // 64
+        TX, TY, LW, LH = d.tile
+        self._temporal_head(f"2D {'Box' if d.neighbourhood == BOX else 'Star'} r={R} "
+                            f"{'Variable' if d.coeff == VARIABLE else 'Jacobi'} "
+                            f"{'Row' if d.orientation == ROW else 'Column'} alignment "
+                            f"{['None', 'Read', 'Write'][d.alignment]}/{d.cl_size} {'f32c' if self.f32 else 'f64'} "
+                            f"temporal: {K} steps per launch, tile {TX}x{TY}, block 64x{BY}", 2 * LW * LH)
+        fn = "static inline" if cpu else "__device__ __forceinline__"
         w(f"#define TCSG_LW {LW}")
         w(f"#define TCSG_LH {LH}")
         w(f"#define TCSG_BUF {LW * LH}")
@@ -1074,7 +1274,7 @@ class _Emitter:
     def source(self, plugin: bool = False) -> str:
         d = self.d
         if d.variant == TEMPORAL:
-            return self.source_temporal()
+            return self.source_temporal3d() if d.dimension == 3 else self.source_temporal()
         self.preamble()
         attr = ""
         if d.variant == STREAM and d.waves_per_eu and self.target != "cpu":

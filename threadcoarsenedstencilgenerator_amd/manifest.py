@@ -112,6 +112,30 @@ def temporal_cases():
     ]
 
 
+def temporal3d_cases():
+    """3D temporal kernels (tsteps per launch, marching through z with a ring of
+    planes per step) with a small grid each: both neighbourhoods, coefficient
+    kinds, orientations, dtypes and alignments; ranges 1-2; 1-3 steps per launch;
+    several tiles in x, y and z, with and without the reference's truncation."""
+    base = dict(dimension=3, variant=TEMPORAL)
+    return [
+        (StencilDesc(neighbourhood=STAR, coeff=VARIABLE, range=1, xc=1, yc=2, zc=6, tsteps=2, max_threads=128,
+                     remainder=True, **base), (150, 13, 20)),
+        (StencilDesc(neighbourhood=BOX, orientation=COLUMN, coeff=JACOBI, range=1, xc=1, yc=1, zc=4, tsteps=2,
+                     max_threads=256, alignment=ALIGN_WRITE, cl_size=8, **base), (140, 11, 15)),
+        (StencilDesc(neighbourhood=STAR, coeff=JACOBI, range=1, xc=1, yc=2, zc=5, tsteps=3, max_threads=128,
+                     alignment=ALIGN_READ, cl_size=16, remainder=True, **base), (135, 14, 23)),
+        (StencilDesc(neighbourhood=BOX, coeff=VARIABLE, range=2, xc=1, yc=1, zc=3, tsteps=1, max_threads=64,
+                     **base), (70, 9, 12)),
+        (StencilDesc(neighbourhood=STAR, orientation=COLUMN, coeff=VARIABLE, range=2, xc=1, yc=2, zc=4, tsteps=2,
+                     max_threads=128, remainder=True, **base), (145, 15, 17)),
+        (StencilDesc(neighbourhood=STAR, coeff=VARIABLE, range=1, xc=1, yc=2, zc=4, tsteps=2, max_threads=128,
+                     dtype=F32C, comp_group=3, remainder=True, **base), (130, 12, 14)),
+        (StencilDesc(neighbourhood=BOX, orientation=COLUMN, coeff=VARIABLE, range=1, xc=1, yc=1, zc=3, tsteps=2,
+                     max_threads=128, dtype=F32C, comp_group=9, **base), (100, 8, 10)),
+    ]
+
+
 def cfg1_desc(shape="plain"):
     """2D Box Jacobi r=1, coarsening 1x1, Row, alignment None."""
     return StencilDesc(dimension=2, neighbourhood=BOX, orientation=ROW, coeff=JACOBI, range=1, xc=1, yc=1,
@@ -200,7 +224,7 @@ def default_build_set():
     (cfg2_sweep, cfg2_tuned, cfg2_f32c_candidates, ...) are compiled on
     demand by the sweeps and cached in the same directory."""
     descs = [d for d, _ in parity_cases()] + [d for d, _ in f32c_cases()] + [d for d, _ in remainder_cases()]
-    descs += [d for d, _ in temporal_cases()]
+    descs += [d for d, _ in temporal_cases()] + [d for d, _ in temporal3d_cases()]
     descs += [cfg1_desc("plain"), cfg1_desc("stream"), cfg1_desc("dppw")]
     descs += [cfg2_desc(1, 1, 1), cfg2_desc(2, 4, 8)]            # reference-shaped kernels
     descs += [smoke_desc()]

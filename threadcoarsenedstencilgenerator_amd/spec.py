@@ -38,10 +38,10 @@ class StencilDesc:
     tuning knobs (max_threads, xext, xcd_remap, nt_store, comp_group,
     prefetch, waves_per_eu, zfirst, batched, nt_load, order, remainder,
     edge_pairs, tsteps) have no reference counterpart: they select the MI355X
-    code shape.  The TEMPORAL variant (2D) advances tsteps time steps per
-    launch: each workgroup stages its tile of (64 xc) x (max_threads/64 yc)
-    points plus tsteps*range halo points per side in LDS and runs the steps
-    there.
+    code shape.  The TEMPORAL variant advances tsteps time steps per launch:
+    each workgroup stages its tile of (64 xc) x (max_threads/64 yc) points
+    plus tsteps*range halo points per side in LDS and runs the steps there
+    (3D: marching through zc planes with a ring of 2r+1 planes per step).
     """
 
     dimension: int = 3
@@ -121,15 +121,13 @@ class StencilDesc:
         if self.variant != TEMPORAL and self.tsteps != 1:
             raise ValueError("tsteps > 1 needs the temporal variant")
         if self.variant == TEMPORAL:
-            if self.dimension != 2:
-                raise ValueError("the temporal variant is built for 2D stencils")
             if not 1 <= self.tsteps <= 16:
                 raise ValueError("tsteps must be in [1, 16]")
             if (self.xext != "load" or self.xcd_remap or self.nt_store or self.nt_load or self.zfirst
                     or self.batched or self.order != "point"):
                 raise ValueError("the temporal variant takes none of the STREAM/PLAIN code-shape options")
-            if self.lds_bytes > 65536:
-                raise ValueError(f"temporal tile needs {self.lds_bytes} bytes of LDS (at most 65536)")
+            if self.lds_bytes > 163840:
+                raise ValueError(f"temporal tile needs {self.lds_bytes} bytes of LDS (gfx950: at most 160 KiB)")
 
     # ---- derived quantities ----
     @property
@@ -156,15 +154,23 @@ class StencilDesc:
     @property
     def tile(self):
         """TEMPORAL: output points per workgroup in x and y, and the LDS tile
-        (outputs plus tsteps*range halo points per side)."""
+        (outputs plus tsteps*range halo points per side).  3D workgroups march
+        through zc planes of such tiles."""
         h = self.tsteps * self.range
         tx, ty = 64 * self.xc, self.max_threads // 64 * self.yc
         return tx, ty, tx + 2 * h, ty + 2 * h
 
     @property
     def lds_bytes(self) -> int:
-        _, _, lw, lh = self.tile
-        return 2 * lw * lh * (4 if self.dtype == F32C else 8)
+        """TEMPORAL: LDS per workgroup.  2D: two buffers of the LDS tile.  3D: a ring of
+        2r+1 planes per level s < tsteps, level s covering the tile plus (tsteps - s) r
+        points per side."""
+        tx, ty, lw, lh = self.tile
+        es = 4 if self.dtype == F32C else 8
+        if self.dimension == 2:
+            return 2 * lw * lh * es
+        r, k = self.range, self.tsteps
+        return sum((2 * r + 1) * (tx + 2 * (k - s) * r) * (ty + 2 * (k - s) * r) for s in range(k)) * es
 
     @property
     def coarsening(self):

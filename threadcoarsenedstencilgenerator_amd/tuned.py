@@ -10,7 +10,7 @@ z-chunk, so no 128-byte line at a tile's x edge is fetched by two XCDs
 from __future__ import annotations
 
 from . import manifest
-from .spec import F32C, F64, TEMPORAL
+from .spec import ALIGN_NONE, F32C, F64, ROW, STAR, STREAM, TEMPORAL, VARIABLE, StencilDesc
 
 # 3D Star Variable r=4, 512^3 interior, FP64, Write alignment at 16 points
 CFG2_F64 = dict(desc=manifest.cfg2_desc(2, 4, 128, shape="dppw", max_threads=256).replace(prefetch=0, zfirst=False),
@@ -61,6 +61,22 @@ BOX2D_R1_ONE_STEP = dict(desc=manifest.cfg1_desc("stream").replace(xc=2, yc=4, m
                          block=(256, 1, 1))
 
 
+# 3D Star Variable r=1 (7 weights) on a 512^3 interior (halo 1), where one step is HBM-bound: the
+# 3D temporal kernel marches each 64x16-point tile through 128 planes, 2 steps per launch, with a ring
+# of 3 planes per step in LDS (61 KB, 512 threads).  Sweep over 1-4 steps per launch, tiles 64x8 to
+# 128x16 and 64x32, 32-128 planes, 256-1024 threads: 2 steps on 64x16x128 was fastest.  The one-step
+# baseline is the fastest one-step shape measured on the same grid (STREAM 2x4x128, x-neighbours by
+# loads; the wave-aligned DPP form ran within 1%).
+STAR3D_R1_GRID = (514, 514, 514)
+STAR3D_R1_TEMPORAL = dict(desc=StencilDesc(dimension=3, neighbourhood=STAR, orientation=ROW, coeff=VARIABLE, range=1,
+                                           xc=1, yc=2, zc=128, alignment=ALIGN_NONE, cl_size=0, variant=TEMPORAL,
+                                           tsteps=2, max_threads=512, remainder=True), block=(64, 8, 1))
+STAR3D_R1_ONE_STEP = dict(desc=StencilDesc(dimension=3, neighbourhood=STAR, orientation=ROW, coeff=VARIABLE, range=1,
+                                           xc=2, yc=4, zc=128, alignment=ALIGN_NONE, cl_size=0, variant=STREAM,
+                                           xext="load", xcd_remap=True, nt_store=True, max_threads=256),
+                          block=(128, 2, 1))
+
+
 def best(workload: str, dtype: int = F64):
     if workload == "star3d_r4_512":
         return CFG2_F32C if dtype == F32C else CFG2_F64
@@ -72,4 +88,5 @@ def best(workload: str, dtype: int = F64):
 def build_set():
     return [CFG2_F64["desc"], CFG2_F32C["desc"], BOX2D_R8_F64["desc"], BOX2D_R8_F32C["desc"],
             CFG2_F64["desc"].replace(remainder=True), CFG2_F32C["desc"].replace(remainder=True),
-            BOX2D_R1_TEMPORAL["desc"], BOX2D_R1_ONE_STEP["desc"], BOX2D_R1_TEMPORAL_F32C["desc"]]
+            BOX2D_R1_TEMPORAL["desc"], BOX2D_R1_ONE_STEP["desc"], BOX2D_R1_TEMPORAL_F32C["desc"],
+            STAR3D_R1_TEMPORAL["desc"], STAR3D_R1_ONE_STEP["desc"]]

@@ -16,9 +16,6 @@ from .spec import ALIGN_NONE, F32C, F64, ROW, STAR, STREAM, TEMPORAL, VARIABLE, 
 CFG2_F64 = dict(desc=manifest.cfg2_desc(2, 4, 128, shape="dppw", max_threads=256).replace(prefetch=0, zfirst=False),
                 block=(128, 2, 1))
 # the same workload in FP32 storage with compensated accumulation
-# (x coarsening 2: FP32 kernels with 4 points per lane in x returned wrong values
-# in lanes 12-15 of each 16-lane group on the test machines; the xc=2 kernels and all
-# FP64 kernels are bit-exact over the full grid)
 # FP32 rows are aligned to 32 points = 128 bytes, the line size the FP64 grid's 16 points
 # make (16 FP32 points, 64 bytes, leaves each row end's line written by two tiles: 5.6%
 # more write traffic); the first group sums the 21 terms up to the output's own plane,
@@ -48,9 +45,8 @@ BOX2D_R1_GRID = (16384 + 2, 16384 + 2)
 # 128x16-point tiles with 512 threads was fastest, 670 GLup/s against 303 for the best one-step shape)
 BOX2D_R1_TEMPORAL = dict(desc=manifest.cfg1_desc("plain").replace(
     variant=TEMPORAL, tsteps=4, xc=2, yc=2, max_threads=512, remainder=True), block=(64, 8, 1))
-# (one-step: STREAM 2x4 with x-neighbours by loads, 324 GLup/s; the segment-DPP form ran at 317.
-# The wave-aligned DPP form, 301, returned wrong points at 16-lane row boundaries in 2 of 3 runs
-# of 8 steps on this grid, where the other two forms were exact in every run.)
+# (one-step: STREAM 2x4 with x-neighbours by loads, 324 GLup/s; the segment-DPP form ran at 317,
+# the wave-aligned DPP form at 301)
 # FP32 storage (the F32C dtype; comp_group 9 puts all 9 terms of the Box r=1 stencil in one group,
 # a plain FP32 chain with no TwoSum): half the LDS per point, so 64x64-point tiles fit; 4 steps
 # per launch, 1169 GLup/s in the sweep

@@ -35,3 +35,11 @@ def test_headline_line_fields():
     assert roof["bound"] == "hbm" and roof["unit"] == "GB/s" and roof["peak"] == 8000.0
     assert abs(roof["frac"] - roof["achieved"] / roof["peak"]) < 1e-3
     assert d["f32c"]["normwise_max_rel"] <= 1e-6
+    # the other measured workloads, summarised in the same line
+    others = d["other_workloads"]
+    assert set(others) == {"box2d_r1_temporal", "star3d_r1_temporal", "box2d_r8", "batched", "star3d_r4_large",
+                           "star3d_r4_511_remainder"}
+    for name, o in others.items():
+        assert "error" not in o, (name, o)
+        assert o["value"] > 0 and o["unit"] == "GLup/s", name
+    assert others["star3d_r1_temporal"]["vs_one_step"] > 1.0

@@ -232,6 +232,8 @@ def main():
     ap.add_argument("--trace", required=True)
     ap.add_argument("--md")
     ap.add_argument("--json")
+    ap.add_argument("--traffic", help="update this JSON (kernel key -> HBM bytes per launch, as bench.py "
+                                      "reads it) with the workloads' FETCH_SIZE/WRITE_SIZE traffic")
     a = ap.parse_args()
     meta = json.load(open(a.meta))
     rep = report(meta, [read_pass(p, meta) for p in a.passes], read_trace(a.trace, meta))
@@ -241,6 +243,23 @@ def main():
         open(a.md, "w").write(md)
     if a.json:
         json.dump(rep, open(a.json, "w"), indent=1)
+    if a.traffic:
+        update_traffic(a.traffic, meta, rep)
+
+
+def update_traffic(path, meta, rep):
+    t = json.load(open(path)) if os.path.exists(path) else {}
+    by_name = {m["name"]: m for m in meta}
+    for r in rep["rows"]:
+        if "hbm_bytes_per_dispatch" not in r:
+            continue
+        m = by_name[r["workload"]]
+        t[m["kernel"]] = {"block": m["workgroup"], "hbm_bytes_per_launch": int(r["hbm_bytes_per_dispatch"]),
+                          "fetch_scale": rep["fetch_scale"], "write_scale": rep["write_scale"],
+                          "algorithmic_bytes_per_launch": m["bytes_per_update"] * m["interior_points"],
+                          "source": "profiles/counters_measured_workloads.md", "workload": m["name"]}
+    with open(path, "w") as f:
+        json.dump(t, f, indent=1)
 
 
 if __name__ == "__main__":

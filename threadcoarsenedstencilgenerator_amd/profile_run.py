@@ -62,8 +62,9 @@ def all_workloads(steps: int, meta: str):
     """One pass over every measured workload (the counter report's dispatches):
     the 2^30-byte calibration copy, then `steps` launches each of the 512^3 FP64
     and FP32C production kernels, the 2D Box r=8 16384x8192 FP64 kernel, the
-    batched persistent kernel over the 64 uneven grids (no graph) and the 2D
-    temporal kernel on 16384^2.  Writes `meta`: per workload the kernel symbol,
+    2D temporal kernel on 16384^2, the 3D temporal and one-step 3D Star r=1
+    kernels on 512^3 and the batched persistent kernel over the 64 uneven grids
+    (no graph).  Writes `meta`: per workload the kernel symbol,
     launch grid and workgroup (threads), interior points, time steps per launch,
     flops per update (reference count) and algorithmic bytes per update."""
     import json
@@ -95,7 +96,11 @@ def all_workloads(steps: int, meta: str):
                                   ("star3d_r4_512_f32c", tuned.CFG2_F32C, (520, 520, 520), rt.INIT_NORMAL),
                                   ("box2d_r8_16384x8192_f64", tuned.BOX2D_R8_F64, tuned.BOX2D_R8_GRID, rt.INIT_UPM1),
                                   ("box2d_r1_16384_temporal_f64", tuned.BOX2D_R1_TEMPORAL, tuned.BOX2D_R1_GRID,
-                                   rt.INIT_U01)):
+                                   rt.INIT_U01),
+                                  ("star3d_r1_512_temporal_f64", tuned.STAR3D_R1_TEMPORAL, tuned.STAR3D_R1_GRID,
+                                   rt.INIT_NORMAL),
+                                  ("star3d_r1_512_one_step_f64", tuned.STAR3D_R1_ONE_STEP, tuned.STAR3D_R1_GRID,
+                                   rt.INIT_NORMAL)):
         d, block = cfg["desc"], tuple(cfg["block"])
         k = rt.Kernel(d)
         if d.coeff == spec.VARIABLE:

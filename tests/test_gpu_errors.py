@@ -52,7 +52,8 @@ def test_launch_argument_errors():
     k.set_mask(rt.mask_seeded(3, d.n))
     # block rows that are not whole waves, and more threads than the launch bound
     assert _status(lambda: k.update(g, (96, 2, 1), rt.grid_blocks(g, (96, 2, 1)), p0, p1)) == BLOCK_SHAPE
-    assert _status(lambda: k.update(g, (128, 4, 1), rt.grid_blocks(g, (128, 4, 1)), p0, p1)) == BLOCK_SHAPE
+    over = (128, 2 * d.max_threads // 128, 1)
+    assert _status(lambda: k.update(g, over, rt.grid_blocks(g, over), p0, p1)) == BLOCK_SHAPE
     # a grid that does not cover the interior, the same buffer as input and output
     short = (blocks[0], blocks[1], blocks[2] - 1)
     assert _status(lambda: k.update(g, block, short, p0, p1)) == ARG

@@ -247,18 +247,20 @@ def bench_box2d_r8(args):
 
 
 def bench_remainder(args):
-    """The 512^3 production kernel generated with remainder handling, on a
-    511^3 interior (no axis a multiple of its coarsening): every point
-    updated, against the same kernel without the option, which truncates
-    that grid as the reference does (z to a multiple of its 128-plane chunk),
-    and against itself on the 512^3 grid (the cost of the store masks)."""
+    """The 512^3 workload's kernel generated with remainder handling (the
+    coarsening and code shape of the production kernel, loading its own rows:
+    LDS staging covers whole coarsening tiles only), on a 511^3 interior (no
+    axis a multiple of its coarsening): every point updated, against the same
+    kernel without the option, which truncates that grid as the reference does
+    (z to a multiple of its 128-plane chunk), and against itself on the 512^3
+    grid (the cost of the store masks)."""
     import torch
 
     from threadcoarsenedstencilgenerator_amd import runtime as rt
     from threadcoarsenedstencilgenerator_amd import tuned
-    best = tuned.best("star3d_r4_512")
+    best = tuned.CFG2_F64_UNSTAGED
     block = tuple(best["block"])
-    kr, kt = rt.Kernel(best["desc"].replace(remainder=True)), rt.Kernel(best["desc"])
+    kr, kt = rt.Kernel(tuned.remainder_cfg()["desc"]), rt.Kernel(best["desc"])
     mask = rt.mask_seeded(MASK_SEED, best["desc"].n)
     stream = torch.cuda.current_stream()
     out = {}
@@ -605,7 +607,8 @@ def main():
                    "coarsening": list(desc.coarsening), "block": list(block),
                    "code_shape": {"variant": "stream" if desc.variant == spec.STREAM else "plain",
                                   "xext": desc.xext, "xcd_remap": desc.xcd_remap, "nt_store": desc.nt_store,
-                                  "max_threads": desc.max_threads},
+                                  "max_threads": desc.max_threads, "edge_pairs": desc.edge_pairs,
+                                  "lds_stage_block": list(desc.stage) if desc.staged else None},
                    "parallelism": "single GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -293,3 +293,81 @@ def test_emulated_temporal3d_kernel_matches_oracle(case):
     ui = np.uint32 if dt == np.float32 else np.uint64
     diff = np.flatnonzero(got.view(ui) != ref.view(ui))
     assert diff.size == 0, f"{diff.size} points differ, first {diff[:5]}"
+
+
+STAGED_CASES = manifest.staged_cases()
+
+
+def _staged_run(d, size, block, edit=None):
+    geo = spec.geometry(d, size)
+    og = oracle.Geometry(3, d.range, d.alignment, d.cl_size, size, d.coarsening)
+    a = np.zeros(og.n_points)
+    oracle.init_seeded(og, 1, 4, a, halo=0.375)
+    mask = oracle.mask_seeded(6, d.n)
+    dt = np.float32 if d.dtype == F32C else np.float64
+    a = a.astype(dt)
+    run = oracle.run_f32c if d.dtype == F32C else oracle.run_f64
+    ref = run(d, og, mask, a.copy(), a.copy(), 2)
+    nblk = tuple((geo.threads[i] + block[i] - 1) // block[i] for i in range(3))
+    got, bad = emulate.run(d, geo, mask.astype(dt), a.copy(), a.copy(), 2, block, nblk, edit=edit)
+    ui = np.uint32 if dt == np.float32 else np.uint64
+    return np.flatnonzero(got.view(ui) != ref.view(ui)), bad, nblk, geo
+
+
+@pytest.mark.parametrize("case", STAGED_CASES, ids=[c[0].key() for c in STAGED_CASES])
+def test_emulated_staged_kernel_matches_oracle(case):
+    """Kernels that stage each plane's rows in LDS (the waves of a workgroup run as host
+    threads meeting at every barrier) equal the oracle bit for bit over 2 steps, with
+    every LDS access inside the tile, of an element written before the last barrier and
+    not written again before the next one."""
+    d, size, block = case
+    diff, bad, nblk, geo = _staged_run(d, size, block)
+    # several blocks per axis, the last one partly outside the interior where a block has several threads
+    assert all(n > 1 for n in nblk) and all(nblk[i] * block[i] > geo.threads[i] for i in range(3) if block[i] > 1)
+    assert bad == 0, f"{bad} bad accesses"
+    assert diff.size == 0, f"{diff.size} points differ, first {diff[:5]}"
+
+
+def _drop_barrier(k):
+    def edit(text):
+        lines = text.split("\n")
+        at = [i for i, l in enumerate(lines) if l.strip() == "__syncthreads();"]
+        del lines[at[k]]
+        return "\n".join(lines)
+    return edit
+
+
+@pytest.mark.parametrize("k", [0, 3])
+def test_emulated_staging_without_a_barrier_is_flagged(k):
+    """With one barrier of a staging kernel removed, a plane's buffer is written while
+    other waves still read it (or read before it is written): the emulation counts
+    those accesses."""
+    d, size, block = STAGED_CASES[0]
+    diff, bad, _, _ = _staged_run(d, size, block, edit=(f"nobarrier{k}", _drop_barrier(k)))
+    assert bad > 0
+
+
+def test_stage_descriptor_limits():
+    base = manifest.cfg2_desc(2, 4, 8, shape="dppw", max_threads=512).replace(prefetch=0)
+    d = base.replace(stage=[128, 4, 1])
+    assert d.stage == (128, 4, 1) and d.staged
+    assert d.stage_lds_bytes == 2 * (4 * 4 + 8) * 128 * 2 * 8
+    # bit 3 marks staging; the block in bits 16-20 (bx/64), 21-25, 26-30
+    f = d.params_blob()[18]
+    assert f & 8 and (f >> 16) & 31 == 2 and (f >> 21) & 31 == 4 and (f >> 26) & 31 == 1
+    assert not base.staged and base.params_blob()[18] & 8 == 0
+    # kernels without staging keep their names
+    assert base.key() == base.replace(stage=(0, 0, 0)).key() != d.key()
+    for bad in (dict(stage=(96, 2, 1)), dict(stage=(128, 4, 2)), dict(stage=(128, 4))):
+        with pytest.raises(ValueError):
+            base.replace(**bad)
+    with pytest.raises(ValueError, match="LDS"):
+        base.replace(stage=(256, 2, 1), yc=8)                 # 2 * 24 * 512 * 8 B > 160 KiB
+    for other in (dict(xext="dpp"), dict(remainder=True), dict(prefetch=1), dict(batched=True)):
+        with pytest.raises(ValueError, match="staging"):
+            base.replace(stage=(64, 2, 1), **other)
+    with pytest.raises(ValueError, match="staging"):
+        manifest.cfg1_desc("dppw").replace(stage=(64, 2, 1))   # 2D
+    from threadcoarsenedstencilgenerator_amd.generator.emit import emit_source
+    with pytest.raises(ValueError, match="block shape"):
+        emit_source(d, plugin=True)

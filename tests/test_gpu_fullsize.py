@@ -10,6 +10,7 @@ import pytest
 
 import oracle
 from threadcoarsenedstencilgenerator_amd import spec, tuned
+from threadcoarsenedstencilgenerator_amd.spec import F32C, F64
 
 pytestmark = pytest.mark.gpu
 GRID = (520, 520, 520)
@@ -53,11 +54,11 @@ def test_production_kernel_bitexact_512(which, steps):
 
 @pytest.mark.parametrize("which", ["f64", "f32c"])
 def test_remainder_kernel_bitexact_511(which):
-    """The production kernels generated with remainder handling on a 511^3
-    interior (no axis a multiple of the 2x4x128 / 2x8x64 coarsening): every
+    """The workload's kernels generated with remainder handling (loading their own rows) on a
+    511^3 interior (no axis a multiple of the 2x4x128 / 2x8x64 coarsening): every
     point updated, bit for bit against the untruncated oracle over 3 steps."""
-    cfg = tuned.CFG2_F64 if which == "f64" else tuned.CFG2_F32C
-    d = cfg["desc"].replace(remainder=True)
+    cfg = tuned.remainder_cfg(F64 if which == "f64" else F32C)
+    d = cfg["desc"]
     grid = (519, 519, 519)
     print(f"full-size remainder {which} {d.key()} grid {grid}", flush=True)
     got, ref = _run(d, tuple(cfg["block"]), 3, grid)

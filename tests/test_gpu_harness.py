@@ -31,12 +31,12 @@ def test_bench_sweep_output_format():
 
 
 def test_bench_single_block_and_mismatch():
-    d = tuned.CFG2_F64["desc"]
-    r = _bench(d, "Star", "3D", "4", "Variable", "Row", "Write", "16", "0.0", "2", "4", "128", "136", "72", "264",
-               "128", "2", "1")
+    d, (bx, by, bz) = tuned.CFG2_F64["desc"], tuned.CFG2_F64["block"]
+    r = _bench(d, "Star", "3D", "4", "Variable", "Row", "Write", "16", "0.0", "2", "4", "128", "264", "72", "264",
+               str(bx), str(by), str(bz))
     assert r.returncode == 0, r.stderr
     rows = [l for l in r.stdout.split("\n") if l and not l.startswith("#")]
-    assert len(rows) == 1 and rows[0].startswith("128\t2\t1\t")
+    assert len(rows) == 1 and rows[0].startswith(f"{bx}\t{by}\t{bz}\t")
     # parameters that differ from the kernel's are refused (the reference's sanity check)
     r = _bench(d, "Star", "3D", "3", "Variable", "Row", "Write", "16", "0.0", "2", "4", "128", "136", "72", "264")
     assert r.returncode != 0 and "different stencil parameters" in r.stderr

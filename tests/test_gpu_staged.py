@@ -1,0 +1,37 @@
+"""STREAM kernels that stage each plane's rows in LDS, on the GPU: bit for bit
+against the oracle on small grids (several blocks per axis, partial last blocks,
+one and two z slices per block, FP32C, Box and Column cases), and the launch
+check that runs such a kernel with the block shape it was generated for only."""
+import pytest
+
+from threadcoarsenedstencilgenerator_amd import manifest
+
+from .parity import assert_bitwise, run_case
+
+pytestmark = pytest.mark.gpu
+
+CASES = manifest.staged_cases()
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0].key() for c in CASES])
+def test_staged_kernel_bitexact(case):
+    d, size, block = case
+    print(f"staged {d.key()} size {size} block {block}", flush=True)
+    got, ref, _, _ = run_case(d, size, steps=3, block=block, mode=2)
+    assert_bitwise(got, ref, f"{d.key()} block {block}")
+
+
+def test_staged_kernel_runs_its_block_only():
+    from threadcoarsenedstencilgenerator_amd import runtime as rt
+    d, size, block = CASES[1]
+    k = rt.Kernel(d)
+    g = k.geometry(size)
+    k.set_mask(rt.mask_seeded(3, d.n))
+    grids = rt.Grids(g, k.dtype)
+    for other in ((128, 2, 1), (64, 4, 1), (128, 4, 2), (256, 2, 1)):
+        print(f"staged {d.key()} refuses block {other}", flush=True)
+        with pytest.raises(rt.TcsgError) as e:
+            k.update(g, other, rt.grid_blocks(g, other), grids.ptr(0), grids.ptr(1))
+        assert e.value.status == 4   # TCSG_ERR_BLOCK_SHAPE
+    k.update(g, block, rt.grid_blocks(g, block), grids.ptr(0), grids.ptr(1))
+    del grids

@@ -245,6 +245,8 @@ struct tcsg_kernel {
     int xext_mode = 0;    // STREAM x-neighbour exchange (params[16]): 0 loads, 1 DPP, 2 wave-aligned DPP
     bool remainder = false;  // kernel generated with remainder handling (params[18] bit 2)
     uint32_t tsteps = 1;     // time steps one launch advances (temporal kernels: params[18] bits 8-15)
+    tcsg_dim3 stage{0, 0, 0};  // STREAM with LDS staging: the block shape it stages for (params[18]
+                               // bit 3; bx/64, by, bz in bits 16-20, 21-25, 26-30), else 0
 };
 
 extern "C" {
@@ -330,6 +332,12 @@ int tcsg_kernel_load(tcsg_kernel **out, const char *path, const tcsg_desc *expec
     k->remainder = (q[18] & 4u) != 0;
     k->tsteps = d.variant == TCSG_TEMPORAL ? (q[18] >> 8) & 0xFFu : 1u;
     if (k->tsteps < 1 || (d.variant == TCSG_TEMPORAL && q[15] % 64)) return fail(TCSG_ERR_MISMATCH);
+    if (q[18] & 8u) {
+        k->stage = tcsg_dim3{((q[18] >> 16) & 31u) * 64u, (q[18] >> 21) & 31u, (q[18] >> 26) & 31u};
+        if (d.variant != TCSG_STREAM || d.dimension != 3 || k->xext_mode != 2 || k->remainder || !k->stage.x ||
+            !k->stage.y || !k->stage.z)
+            return fail(TCSG_ERR_MISMATCH);
+    }
     *out = k;
     return TCSG_OK;
 }
@@ -404,6 +412,8 @@ int tcsg_check_block(const tcsg_kernel *k, tcsg_dim3 b)
     // a temporal kernel's LDS tile is sized for its compiled block (64, launch bound / 64, 1)
     if (k->desc.variant == TCSG_TEMPORAL && (b.x != 64 || b.y != k->params[15] / 64 || b.z != 1))
         return TCSG_ERR_BLOCK_SHAPE;
+    // an LDS-staging kernel's tile is sized and indexed for the block it was generated for
+    if (k->stage.x && (b.x != k->stage.x || b.y != k->stage.y || b.z != k->stage.z)) return TCSG_ERR_BLOCK_SHAPE;
     if (k->xext_mode == 1) {
         // every run of lanes in one block row and one wave must span the
         // H = ceil(r/xc) lanes the x-neighbour exchange reaches
@@ -436,13 +446,16 @@ struct StreamLimits {
     uint32_t rec, oob;
 };
 
-static int stream_limits(const tcsg_desc &d, const tcsg_geometry *g, uint32_t bext, StreamLimits *out)
+static int stream_limits(const tcsg_desc &d, const tcsg_geometry *g, uint32_t bext, uint32_t brows, StreamLimits *out)
 {
     const uint64_t es = elem_size(d);
     const uint64_t sx = (uint64_t)g->size.x * es, sxy = sx * g->size.y;
     const uint64_t H = (d.range + d.xc - 1) / d.xc;
     const uint64_t cols = (2 * H * d.xc + d.xc + 2ull * d.range) * es;
-    const uint64_t span = (d.zc + 2ull * d.range) * (d.dimension == 3 ? sxy : 0) + (d.yc + 2ull * d.range) * sx + cols;
+    // rows a lane's loads reach from its base: its footprint's (brows = 1), or with LDS staging
+    // those of its block's brows thread rows
+    const uint64_t rows = (uint64_t)brows * d.yc + 2ull * d.range;
+    const uint64_t span = (d.zc + 2ull * d.range) * (d.dimension == 3 ? sxy : 0) + rows * sx + cols;
     const uint64_t window = d.dimension == 3 ? ((uint64_t)bext * d.zc + 2ull * d.range) * sxy
                                              : ((uint64_t)bext * d.yc + 2ull * d.range) * sx;
     const uint64_t slack = sx + cols;
@@ -463,15 +476,17 @@ static int check_geom(const tcsg_desc &d, const tcsg_geometry *g, bool remainder
         ref.left_offset != g->left_offset || ref.right_offset != g->right_offset || ref.n_points != g->n_points)
         return TCSG_ERR_ARG;
     // (the smallest block window: launches check theirs)
-    if (d.variant == TCSG_STREAM) return stream_limits(d, g, 1, nullptr);
+    if (d.variant == TCSG_STREAM) return stream_limits(d, g, 1, 1, nullptr);
     return TCSG_OK;
 }
 
 // Block extent along the axis a STREAM kernel's buffer window covers.
 static uint32_t window_extent(const tcsg_desc &d, tcsg_dim3 block) { return d.dimension == 3 ? block.z : block.y; }
+// thread rows whose footprint rows one lane's loads cover (more than its own with LDS staging)
+static uint32_t load_rows(const tcsg_kernel *k, tcsg_dim3 block) { return k->stage.x ? block.y : 1u; }
 
-static void fill_kargs(const tcsg_desc &d, const tcsg_geometry *g, tcsg_dim3 block, const void *in, void *out,
-                       const void *mask, KArgs &a)
+static void fill_kargs(const tcsg_desc &d, const tcsg_geometry *g, tcsg_dim3 block, uint32_t brows, const void *in,
+                       void *out, const void *mask, KArgs &a)
 {
     a = KArgs{};
     a.in = in;
@@ -485,7 +500,7 @@ static void fill_kargs(const tcsg_desc &d, const tcsg_geometry *g, tcsg_dim3 blo
     a.rec_limit = 0xFFFFFFFFu;
     a.oob = 0xFFFFFFFFu;
     StreamLimits lim;
-    if (d.variant == TCSG_STREAM && stream_limits(d, g, window_extent(d, block), &lim) == TCSG_OK) {
+    if (d.variant == TCSG_STREAM && stream_limits(d, g, window_extent(d, block), brows, &lim) == TCSG_OK) {
         a.rec_limit = lim.rec;
         a.oob = lim.oob;
     }
@@ -537,7 +552,7 @@ static int launch_one(tcsg_kernel *k, const tcsg_geometry *g, tcsg_dim3 block, t
                       void *out, hipStream_t s)
 {
     KArgs a;
-    fill_kargs(k->desc, g, block, in, out, k->mask, a);
+    fill_kargs(k->desc, g, block, load_rows(k, block), in, out, k->mask, a);
     size_t sz = sizeof(a);
     void *extra[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, &a, HIP_LAUNCH_PARAM_BUFFER_SIZE, &sz, HIP_LAUNCH_PARAM_END};
     const hipError_t e = hipModuleLaunchKernel(k->fn, blocks.x, blocks.y, blocks.z, block.x, block.y, block.z, 0, s,
@@ -558,7 +573,7 @@ static int precheck(tcsg_kernel *k, const tcsg_geometry *g, tcsg_dim3 block, tcs
     rc = tcsg_check_block(k, block);
     if (rc) return rc;
     if (k->desc.variant == TCSG_STREAM) {
-        rc = stream_limits(k->desc, g, window_extent(k->desc, block), nullptr);
+        rc = stream_limits(k->desc, g, window_extent(k->desc, block), load_rows(k, block), nullptr);
         if (rc) return rc;
     }
     if (!blocks.x || !blocks.y || !blocks.z) return TCSG_ERR_ARG;
@@ -723,7 +738,7 @@ int tcsg_batch_create(tcsg_batch **out, tcsg_kernel *k, uint32_t n, const tcsg_g
     for (uint32_t i = 0; i < n; ++i) {
         rc = check_geom(k->desc, &geoms[i], k->remainder);
         if (!rc && k->desc.variant == TCSG_STREAM)
-            rc = stream_limits(k->desc, &geoms[i], window_extent(k->desc, block), nullptr);
+            rc = stream_limits(k->desc, &geoms[i], window_extent(k->desc, block), load_rows(k, block), nullptr);
         if (!rc && (!bufs0[i] || !bufs1[i] || bufs0[i] == bufs1[i])) rc = TCSG_ERR_ARG;
         if (rc) { delete b; return rc; }
         const tcsg_dim3 nb = tcsg_grid_blocks(&geoms[i], block);
@@ -734,7 +749,8 @@ int tcsg_batch_create(tcsg_batch **out, tcsg_kernel *k, uint32_t n, const tcsg_g
         b->points += (uint64_t)geoms[i].elements.x * geoms[i].elements.y * geoms[i].elements.z;
         for (int p = 0; p < 2; ++p) {
             KArgs a;
-            fill_kargs(k->desc, &geoms[i], block, p ? bufs1[i] : bufs0[i], p ? bufs0[i] : bufs1[i], k->mask, a);
+            fill_kargs(k->desc, &geoms[i], block, load_rows(k, block), p ? bufs1[i] : bufs0[i],
+                       p ? bufs0[i] : bufs1[i], k->mask, a);
             args[p].push_back(a);
         }
         for (uint32_t z = 0; z < nb.z; ++z)

@@ -15,7 +15,10 @@ shape (no reference counterpart) are given as --name=value:
 --order=point|offset (plain variant), --no-xcd-remap, --no-nt-store, --zfirst,
 --remainder (grids whose interior is not a multiple of the coarsening are
 updated whole instead of truncated), --edge-pairs (--xext=dppw: the wave-edge
-lanes fetch two adjacent rows' extra columns per load).
+lanes fetch two adjacent rows' extra columns per load), --stage=BXxBYxBZ (3D,
+--xext=dppw: LDS halo staging for blocks of BX x BY x BZ threads, the only
+block shape such a kernel is then launched with; its source has no reference
+updateGrid, which takes any block shape).
 Invalid arguments print the usage on stderr and exit with status 1, as the
 reference does.
 """
@@ -106,6 +109,11 @@ def desc_from_argv(dim: int, argv):
                  ("comp-group", "comp_group"), ("tsteps", "tsteps")):
         if k in opts:
             kw[f] = _int(opts[k], k, usage)
+    if "stage" in opts:
+        parts = opts["stage"].split("x")
+        if len(parts) != 3:
+            _err("--stage takes a block shape BXxBYxBZ", usage)
+        kw["stage"] = tuple(_int(v, "stage", usage) for v in parts)
     if kw["variant"] == spec.PLAIN:
         kw.update(xext="load", xcd_remap=False, nt_store=False, zfirst=False, order=opts.get("order", "point"))
     elif kw["variant"] == spec.TEMPORAL:
@@ -131,9 +139,11 @@ def generate(dim: int, argv=None) -> int:
             src = os.path.join(td, "kernels.hip")
             open(src, "w").write(emit_source(d))
             _run([HIPCC, *HIP_FLAGS, "--genco", "--no-gpu-bundle-output", src, "-o", out])
+        from .. import isa_check
+        isa_check.check_code_object(out)
     else:
         # the printed kernels.cu also carries the reference kernel file's symbols (FP64)
-        sys.stdout.write(emit_source(d, plugin=d.dtype == spec.F64 and d.variant != spec.TEMPORAL))
+        sys.stdout.write(emit_source(d, plugin=d.dtype == spec.F64 and d.variant != spec.TEMPORAL and not d.staged))
     return 0
 
 

@@ -12,9 +12,16 @@
 // does not fit in 32 bits is counted in tcsg_emu_bad_access (its hardware
 // meaning would depend on wrap-around); tests require the count to stay 0.
 #pragma once
+#include <atomic>
 #include <cmath>
+#include <condition_variable>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
+#include <functional>
+#include <memory>
+#include <mutex>
+#include <pthread.h>
 #include <type_traits>
 #include <vector>
 
@@ -179,6 +186,8 @@ inline tcsg_emu_rsrc tcsg_make_rsrc(const void *p, uint32_t bytes) { return {(ch
 extern "C" {
 extern unsigned long tcsg_emu_bad_access;
 }
+// counted atomically: the waves of an LDS-staging kernel's workgroup run as host threads
+inline void tcsg_emu_bad() { __atomic_fetch_add(&tcsg_emu_bad_access, 1ul, __ATOMIC_RELAXED); }
 
 // A value the hardware holds in scalar registers (wave-uniform): lane 0's value; a lane
 // that disagrees is counted in tcsg_emu_bad_access.
@@ -186,7 +195,7 @@ template <class T>
 T tcsg_uni(const Lane<T> &v)
 {
     for (int i = 1; i < tcsg_emu::W; ++i)
-        if (v.v[i] != v.v[0]) ++tcsg_emu_bad_access;
+        if (v.v[i] != v.v[0]) tcsg_emu_bad();
     return v.v[0];
 }
 template <class T, class = std::enable_if_t<std::is_arithmetic<T>::value>>
@@ -198,7 +207,7 @@ Lane<T> tcsg_emu_ld(const tcsg_emu_rsrc &r, const u32 &off, const u32 &soff)
     Lane<T> out(0);
     for (int i = 0; i < tcsg_emu::W; ++i) {
         const uint64_t full = (uint64_t)off.v[i] + soff.v[i];
-        if (full >> 32) { ++tcsg_emu_bad_access; continue; }
+        if (full >> 32) { tcsg_emu_bad(); continue; }
         if (full + sizeof(T) > r.records) continue;  // out of range: reads 0
         std::memcpy(&out.v[i], r.base + full, sizeof(T));
     }
@@ -210,7 +219,7 @@ void tcsg_emu_st(const tcsg_emu_rsrc &r, const u32 &off, const u32 &soff, const 
 {
     for (int i = 0; i < tcsg_emu::W; ++i) {
         const uint64_t full = (uint64_t)off.v[i] + soff.v[i];
-        if (full >> 32) { ++tcsg_emu_bad_access; continue; }
+        if (full >> 32) { tcsg_emu_bad(); continue; }
         if (full + sizeof(T) > r.records) continue;  // out of range: dropped
         std::memcpy(r.base + full, &v.v[i], sizeof(T));
     }
@@ -220,4 +229,114 @@ void tcsg_emu_st(const tcsg_emu_rsrc &r, const u32 &off, const u32 &soff, const 
 struct tcsg_emu_dim3u { uint32_t x, y, z; };
 struct tcsg_emu_dim3l { u32 x, y, z; };
 extern tcsg_emu_dim3u blockIdx, blockDim, gridDim;
-extern tcsg_emu_dim3l threadIdx;
+extern thread_local tcsg_emu_dim3l threadIdx;
+
+// Workgroups of LDS-staging kernels: each wave runs on its own host thread and the waves
+// meet at __syncthreads.  tcsg_emu_phase counts the barriers a wave has passed, so two
+// accesses of one LDS element in the same phase by different waves, or a write and any
+// other access in one phase, are a race the barriers do not order; such accesses, and
+// ones outside the LDS or of elements never written, are counted as bad accesses.
+extern thread_local unsigned long tcsg_emu_phase;
+
+struct tcsg_emu_barrier_t {
+    std::mutex m;
+    std::condition_variable cv;
+    unsigned n = 1, count = 0;
+    unsigned long gen = 0;
+    void start(unsigned waves) { std::lock_guard<std::mutex> l(m); n = waves; count = 0; }
+};
+inline tcsg_emu_barrier_t &tcsg_emu_bar()
+{
+    static tcsg_emu_barrier_t b;
+    return b;
+}
+inline void __syncthreads()
+{
+    tcsg_emu_barrier_t &b = tcsg_emu_bar();
+    std::unique_lock<std::mutex> l(b.m);
+    const unsigned long g = b.gen;
+    if (++b.count == b.n) {
+        b.count = 0;
+        ++b.gen;
+        b.cv.notify_all();
+    } else {
+        b.cv.wait(l, [&] { return b.gen != g; });
+    }
+    ++tcsg_emu_phase;
+}
+
+// Runs fn(0..n-1) on n host threads with stacks large enough for the emulated waves'
+// straight-line kernels (every per-lane value is a 64-lane array on the stack).
+inline void tcsg_emu_run_waves(unsigned n, const std::function<void(unsigned)> &fn)
+{
+    struct Arg {
+        const std::function<void(unsigned)> *fn;
+        unsigned w;
+    };
+    std::vector<pthread_t> th(n);
+    std::vector<Arg> args(n);
+    pthread_attr_t at;
+    pthread_attr_init(&at);
+    pthread_attr_setstacksize(&at, (size_t)1 << 30);
+    for (unsigned i = 0; i < n; ++i) {
+        args[i] = Arg{&fn, i};
+        if (pthread_create(&th[i], &at, [](void *p) -> void * {
+                const Arg *a = (const Arg *)p;
+                (*a->fn)(a->w);
+                return nullptr;
+            }, &args[i]) != 0)
+            std::abort();
+    }
+    for (auto &t : th) pthread_join(t, nullptr);
+    pthread_attr_destroy(&at);
+}
+
+template <class T>
+struct tcsg_emu_lds {
+    std::vector<T> v;
+    // per element: phase + 1 of the last write and of the last read (0: none), and the
+    // wave (its first lane's flat thread id) of the last write
+    std::unique_ptr<std::atomic<unsigned long>[]> wph, rph, wwave;
+    size_t n;
+    explicit tcsg_emu_lds(size_t n_)
+        : v(n_), wph(new std::atomic<unsigned long>[n_]), rph(new std::atomic<unsigned long>[n_]),
+          wwave(new std::atomic<unsigned long>[n_]), n(n_) { reset(); }
+    void reset()
+    {
+        for (size_t i = 0; i < n; ++i) { wph[i] = 0; rph[i] = 0; wwave[i] = 0; }
+    }
+};
+inline unsigned long tcsg_emu_wave_id()
+{
+    return threadIdx.x.v[0] + blockDim.x * (threadIdx.y.v[0] + blockDim.y * threadIdx.z.v[0]);
+}
+template <class T>
+Lane<T> tcsg_lds_ld(tcsg_emu_lds<T> &l, const u32 &idx)
+{
+    Lane<T> out(0);
+    const unsigned long ph = tcsg_emu_phase + 1;
+    for (int i = 0; i < tcsg_emu::W; ++i) {
+        const uint32_t e = idx.v[i];
+        if (e >= l.n) { tcsg_emu_bad(); continue; }
+        const unsigned long wp = l.wph[e].load(std::memory_order_relaxed);
+        if (wp == 0 || wp == ph) tcsg_emu_bad();   // never written, or written in this phase
+        l.rph[e].store(ph, std::memory_order_relaxed);
+        out.v[i] = l.v[e];
+    }
+    return out;
+}
+template <class T>
+void tcsg_lds_st(tcsg_emu_lds<T> &l, const u32 &idx, const Lane<T> &val)
+{
+    const unsigned long ph = tcsg_emu_phase + 1, wave = tcsg_emu_wave_id();
+    for (int i = 0; i < tcsg_emu::W; ++i) {
+        const uint32_t e = idx.v[i];
+        if (e >= l.n) { tcsg_emu_bad(); continue; }
+        if (l.rph[e].load(std::memory_order_relaxed) == ph) tcsg_emu_bad();   // read in this phase
+        if (l.wph[e].load(std::memory_order_relaxed) == ph && l.wwave[e].load(std::memory_order_relaxed) != wave)
+            tcsg_emu_bad();                                                    // another wave wrote it
+        l.wph[e].store(ph, std::memory_order_relaxed);
+        l.wwave[e].store(wave, std::memory_order_relaxed);
+        l.v[e] = val.v[i];
+    }
+}

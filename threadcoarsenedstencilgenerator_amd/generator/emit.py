@@ -250,6 +250,15 @@ class _Emitter:
                     w(f"  return __hiloint2double(__builtin_amdgcn_mov_dpp(hi, {ctrl:#x}, 0xf, 0xf, true),")
                     w(f"                          __builtin_amdgcn_mov_dpp(lo, {ctrl:#x}, 0xf, 0xf, true));")
                     w("}")
+        if d.variant == STREAM and d.staged:
+            bx, by, bz = d.stage
+            n_lds = bz * 2 * (by * d.yc + 2 * d.range) * bx * d.xc
+            if cpu:
+                w(f"// the workgroup's LDS ({n_lds} elements); accesses are range- and race-checked")
+                w(f"static tcsg_emu_lds<{T}> tcsg_emu_stage({n_lds});")
+            else:
+                w(f"__device__ __forceinline__ {T} tcsg_lds_ld(const {T}* p, uint32_t i) {{ return p[i]; }}")
+                w(f"__device__ __forceinline__ void tcsg_lds_st({T}* p, uint32_t i, {T} v) {{ p[i] = v; }}")
         if d.variant == STREAM:
             w("// Buffer access: per-lane byte offset `off` (range-checked: lanes given an")
             w("// offset past the buffer read 0 and their stores are dropped) plus a")
@@ -322,7 +331,8 @@ class _Emitter:
         """Host loop that runs every wave of a launch through the emulated kernel."""
         w = self.w
         w("tcsg_emu_dim3u blockIdx, blockDim, gridDim;")
-        w("tcsg_emu_dim3l threadIdx;")
+        w("thread_local tcsg_emu_dim3l threadIdx;")
+        w("thread_local unsigned long tcsg_emu_phase = 0;")
         w('extern "C" { unsigned long tcsg_emu_bad_access = 0; }')
         w('extern "C" int tcsg_emulate(const tcsg_kargs* a, uint32_t gx, uint32_t gy, uint32_t gz,')
         w("                            uint32_t bx, uint32_t by, uint32_t bz) {")
@@ -332,14 +342,28 @@ class _Emitter:
         w("  blockDim = {bx, by, bz};")
         w("  for (uint32_t k = 0; k < gz; ++k) for (uint32_t j = 0; j < gy; ++j) for (uint32_t i = 0; i < gx; ++i) {")
         w("    blockIdx = {i, j, k};")
-        w("    for (uint32_t w0 = 0; w0 < nt; w0 += 64) {")
-        w("      for (int l = 0; l < 64; ++l) {")
-        w("        const uint32_t lid = w0 + l;")
-        w("        threadIdx.x.v[l] = lid % bx; threadIdx.y.v[l] = (lid / bx) % by; threadIdx.z.v[l] = lid / (bx * by);")
-        w("      }")
-        w("      tcsg_update(*a);")
-        w("    }")
-        w("  }")
+        if self.d.variant == STREAM and self.d.staged:
+            w("    // the waves of the workgroup run as host threads that meet at each __syncthreads")
+            w("    tcsg_emu_stage.reset();")
+            w("    tcsg_emu_bar().start(nt / 64u);")
+            w("    tcsg_emu_run_waves(nt / 64u, [=](unsigned wv) {")
+            w("      for (int l = 0; l < 64; ++l) {")
+            w("        const uint32_t lid = wv * 64u + l;")
+            w("        threadIdx.x.v[l] = lid % bx; threadIdx.y.v[l] = (lid / bx) % by; threadIdx.z.v[l] = lid / (bx * by);")
+            w("      }")
+            w("      tcsg_emu_phase = 0;")
+            w("      tcsg_update(*a);")
+            w("    });")
+            w("  }")
+        else:
+            w("    for (uint32_t w0 = 0; w0 < nt; w0 += 64) {")
+            w("      for (int l = 0; l < 64; ++l) {")
+            w("        const uint32_t lid = w0 + l;")
+            w("        threadIdx.x.v[l] = lid % bx; threadIdx.y.v[l] = (lid / bx) % by; threadIdx.z.v[l] = lid / (bx * by);")
+            w("      }")
+            w("      tcsg_update(*a);")
+            w("    }")
+            w("  }")
         w("  return 0;")
         w("}")
         if self.d.batched:
@@ -641,6 +665,28 @@ class _Emitter:
         both_edges = dpp and wave_aligned and merged_left and 2 * H <= 64
         if both_edges:
             w("  const u32 eoffE = tcsg_sel(edgeL, eoffL, eoffR);")
+        staged = d.staged
+        if staged:
+            BX, BY, BZ = d.stage
+            TX, TY = BX * XC, BY * self.YC
+            NROW = TY + 2 * R
+            NR = -(-NROW // BY)
+            w(f"  // LDS halo staging (blocks of {BX}x{BY}x{BZ} threads): each z-slice of the block loads")
+            w(f"  // every plane's {NROW} rows ({TY} rows and {R} halo rows per side) by {TX} columns once")
+            w(f"  // into one of two LDS buffers, thread row ty the tile rows ty + {BY} i, and each thread")
+            w("  // reads its footprint rows there.  Plane p goes to buffer p % 2: its rows are loaded")
+            w("  // two planes ahead, written to LDS after the plane before it is done, and read after")
+            w("  // the barrier that follows; the barrier after plane p's reads frees its buffer.")
+            if not self.target == "cpu":
+                w(f"  __shared__ TV tcsg_stage[{BZ * 2 * NROW * TX}];")
+            else:
+                w("  auto& tcsg_stage = tcsg_emu_stage;")
+            w(f"  const u32 yb = byi * {TY}u;  // the block's first footprint row")
+            w(f"  const u32 sread = (threadIdx.z * {2 * NROW}u + threadIdx.y * {self.YC}u) * {TX}u + threadIdx.x * {XC}u;")
+            w(f"  const u32 swrite = (threadIdx.z * {2 * NROW}u + threadIdx.y) * {TX}u + threadIdx.x * {XC}u;")
+            w("  // any row and column of the tile can feed a valid output: every lane of a valid z chunk loads")
+            w(f"  const u32 stbase = tcsg_sel(gz < thz, ((z0 - wb) * ydim + yb + threadIdx.y) * xdim * {es}u "
+              f"+ (x0 - {XL}u) * {es}u, oob);")
         pts, contrib = self.footprint()
         # edge_pairs: two adjacent rows of one plane that both need edge columns share a load;
         # the second row's columns go to the next H lanes on each side (one row below in the
@@ -746,25 +792,52 @@ class _Emitter:
                         out.append((ename(side, 0, ip, py2, pz2), off, soff(py2, pz2)))
                     else:
                         qx, qy, qz = sp
+                        vcoord[vname(qx, qy, qz)] = sp
                         out.append((vname(qx, qy, qz), f"base + {(qx + XL) * es}u", soff(qy, qz)))
             return out
 
+        vcoord = {}
+
         declared = set()
 
-        def emit_loads(k):
+        def emit_loads(k, buf=0):
             for nm, off, so in loads_of(k):
                 if nm in declared:
                     continue
                 declared.add(nm)
+                if staged and nm in vcoord:
+                    qx, qy, _ = vcoord[nm]
+                    w(f"  const TV {nm} = tcsg_lds_ld(tcsg_stage, sread + {(buf * NROW + qy + R) * TX + qx}u);")
+                    continue
                 w(f"  const TV {nm} = tcsg_ld(rin, {off}, {so});")
 
+        def stage_loads(gi):
+            for i in range(NR):
+                for c in range(XC):
+                    w(f"  const TV sg{gi}_{i}_{c} = tcsg_ld(rin, stbase + {c * es}u, "
+                      f"({i * BY}u * sx + {order[gi] + ZR}u * sxy));")
+
+        def stage_writes(gi):
+            for i in range(NR):
+                for c in range(XC):
+                    st = f"tcsg_lds_st(tcsg_stage, swrite + {((gi % 2) * NROW + i * BY) * TX + c}u, sg{gi}_{i}_{c});"
+                    # (thread rows are whole waves: the condition is uniform across each wave)
+                    w(f"  if (tcsg_uni(threadIdx.y) + {i * BY}u < {NROW}u) {st}" if (i + 1) * BY > NROW else f"  {st}")
+
         D = d.prefetch
+        if staged:
+            assert order == sorted(order) and D == 0
+            stage_loads(0)
+            stage_writes(0)
+            w("  __syncthreads();")
+            if len(order) > 1:
+                stage_loads(1)
         for i in range(min(D, len(order))):
             emit_loads(order[i])
         for gi, k in enumerate(order):
             if gi + D < len(order):
-                emit_loads(order[gi + D])
-            emit_loads(k)
+                emit_loads(order[gi + D], (gi + D) % 2)
+            emit_loads(k, gi % 2)
             gpts = [p for _, p in members[k]]
             if dpp:
                 # x-neighbours: main lane-shift chain from the own columns, halo
@@ -832,6 +905,11 @@ class _Emitter:
                             c0 += n
                         for c in range(XC - 1):
                             w(f"  tcsg_st({rs}, spart{c}, {so}, 0u, {row[c]});")
+            if staged and gi + 1 < len(order):
+                stage_writes(gi + 1)
+                w("  __syncthreads();")
+                if gi + 2 < len(order):
+                    stage_loads(gi + 2)
 
     # ------------------------------------------------------------------ all
     # ------------------------------------------------------------- TEMPORAL
@@ -1530,6 +1608,8 @@ def emit_source(desc: StencilDesc, target: str = "gfx950", plugin: bool = False)
         raise ValueError("CPU emulation covers the STREAM and TEMPORAL variants")
     if plugin and desc.variant == TEMPORAL:
         raise ValueError("the reference's updateGrid advances one step per launch; a temporal kernel has no such symbol")
+    if plugin and desc.staged:
+        raise ValueError("the reference's updateGrid takes any block shape; an LDS-staging kernel runs one")
     if target not in ("gfx950", "cpu"):
         raise ValueError(f"unknown target {target}")
     return _Emitter(desc, target).source(plugin)

@@ -27,23 +27,31 @@ class _KArgs(C.Structure):
                 ("in_bytes", C.c_uint64), ("out_bytes", C.c_uint64), ("rec_limit", C.c_uint32), ("oob", C.c_uint32)]
 
 
-def build(desc: spec.StencilDesc) -> str:
+def build(desc: spec.StencilDesc, edit=None) -> str:
+    """Build (or reuse) the emulation library of ``desc``; ``edit``, a (name, function)
+    pair, builds a separate library from the emitted text as the function rewrites it
+    (tests use it to check that the emulation detects a broken kernel)."""
     os.makedirs(EMU_DIR, exist_ok=True)
-    so = os.path.join(EMU_DIR, desc.key() + ".so")
+    so = os.path.join(EMU_DIR, desc.key() + (f"_{edit[0]}" if edit else "") + ".so")
     src_text = emit_source(desc, "cpu")
+    if edit:
+        src_text = edit[1](src_text)
+    # the stamp covers the emulation header too, so a changed header rebuilds
+    with open(os.path.join(HERE, "cpu_emu.h")) as f:
+        stamp_text = src_text + "\n// cpu_emu.h\n" + f.read()
     stamp = so + ".src"
-    if os.path.exists(so) and os.path.exists(stamp) and open(stamp).read() == src_text:
+    if os.path.exists(so) and os.path.exists(stamp) and open(stamp).read() == stamp_text:
         return so
     with tempfile.TemporaryDirectory(prefix="tcsg_emu_") as td:
         src = os.path.join(td, "k.cpp")
         open(src, "w").write(src_text)
         tmp = so + f".{os.getpid()}.tmp"
-        r = subprocess.run(["g++", "-O1", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", f"-I{HERE}",
+        r = subprocess.run(["g++", "-O1", "-std=c++17", "-pthread", "-fPIC", "-shared", "-ffp-contract=off", f"-I{HERE}",
                             src, "-o", tmp], capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"emulation build failed for {desc.key()}:\n{r.stderr[-4000:]}")
         os.replace(tmp, so)
-    open(stamp, "w").write(src_text)
+    open(stamp, "w").write(stamp_text)
     return so
 
 
@@ -59,17 +67,21 @@ def stream_limits(desc: spec.StencilDesc, geom, bext: int):
     r = desc.range
     h = (r + desc.xc - 1) // desc.xc
     cols = (2 * h * desc.xc + desc.xc + 2 * r) * es
-    span = (desc.zc + 2 * r) * (sxy if desc.dimension == 3 else 0) + (desc.yc + 2 * r) * sx + cols
+    # rows a lane's loads reach from its base: its footprint's, or with LDS staging the block's
+    rows = (desc.stage[1] if desc.staged else 1) * desc.yc + 2 * r
+    span = (desc.zc + 2 * r) * (sxy if desc.dimension == 3 else 0) + rows * sx + cols
     window = (bext * desc.zc + 2 * r) * sxy if desc.dimension == 3 else (bext * desc.yc + 2 * r) * sx
     slack = sx + cols
     assert window + 2 * slack + span < 1 << 32
     return window + slack, window + 2 * slack
 
 
-def run(desc: spec.StencilDesc, geom, mask, a: np.ndarray, b: np.ndarray, steps: int, block, blocks):
+def run(desc: spec.StencilDesc, geom, mask, a: np.ndarray, b: np.ndarray, steps: int, block, blocks, edit=None):
     """`steps` ping-pong steps of the emulated kernel on host arrays a/b
-    (geometry as from spec.geometry).  Returns (final array, bad accesses)."""
-    L = C.CDLL(build(desc))
+    (geometry as from spec.geometry).  Returns (final array, bad accesses of this
+    call)."""
+    L = C.CDLL(build(desc, edit))
+    bad0 = C.c_ulong.in_dll(L, "tcsg_emu_bad_access").value
     L.tcsg_emulate.argtypes = [C.c_void_p] + [C.c_uint32] * 6
     dt = np.float32 if desc.dtype == F32C else np.float64
     m = np.ascontiguousarray(mask if mask is not None else np.zeros(1), dtype=dt)
@@ -82,7 +94,7 @@ def run(desc: spec.StencilDesc, geom, mask, a: np.ndarray, b: np.ndarray, steps:
         rc = L.tcsg_emulate(C.byref(k), *blocks, *block)
         if rc != 0:
             raise ValueError("block shape must be whole waves")
-    bad = C.c_ulong.in_dll(L, "tcsg_emu_bad_access").value
+    bad = C.c_ulong.in_dll(L, "tcsg_emu_bad_access").value - bad0
     return bufs[steps & 1], bad
 
 

@@ -214,6 +214,36 @@ def store_hazard_cases():
     ]
 
 
+def staged_cases():
+    """(desc, grid size incl. halo, block) of STREAM kernels with LDS halo staging on small
+    grids: several blocks per axis with partial last blocks, block rows of one and two waves,
+    one and two z slices per block, thread rows that do not divide the tile's row count (the
+    last staging load of some thread rows is not stored), FP32C, Box and Column cases, edge
+    columns one or two rows per load."""
+    dppw = dict(variant=STREAM, xext="dppw", xcd_remap=True, nt_store=True, prefetch=0)
+    return [
+        (StencilDesc(dimension=3, neighbourhood=STAR, orientation=ROW, coeff=VARIABLE, range=4, xc=2, yc=4, zc=4,
+                     alignment=ALIGN_WRITE, cl_size=16, edge_pairs=True, stage=(64, 2, 2), max_threads=256, **dppw),
+         (222, 28, 21), (64, 2, 2)),
+        (StencilDesc(dimension=3, neighbourhood=STAR, orientation=ROW, coeff=VARIABLE, range=4, xc=2, yc=4, zc=3,
+                     alignment=ALIGN_WRITE, cl_size=16, stage=(128, 4, 1), max_threads=512, **dppw),
+         (300, 45, 18), (128, 4, 1)),
+        (StencilDesc(dimension=3, neighbourhood=BOX, orientation=ROW, coeff=JACOBI, range=2, xc=2, yc=2, zc=3,
+                     alignment=ALIGN_NONE, cl_size=0, stage=(64, 3, 1), max_threads=256, **dppw),
+         (150, 19, 14), (64, 3, 1)),
+        (StencilDesc(dimension=3, neighbourhood=STAR, orientation=ROW, coeff=VARIABLE, range=2, xc=4, yc=2, zc=2,
+                     alignment=ALIGN_WRITE, cl_size=32, dtype=F32C, comp_group=3, edge_pairs=True, stage=(64, 2, 1),
+                     max_threads=256, **dppw),
+         (300, 15, 12), (64, 2, 1)),
+        (StencilDesc(dimension=3, neighbourhood=STAR, orientation=COLUMN, coeff=JACOBI, range=1, xc=1, yc=1, zc=4,
+                     alignment=ALIGN_READ, cl_size=8, stage=(128, 1, 1), max_threads=256, **dppw),
+         (200, 9, 11), (128, 1, 1)),
+        (StencilDesc(dimension=3, neighbourhood=BOX, orientation=ROW, coeff=VARIABLE, range=1, xc=2, yc=2, zc=2,
+                     alignment=ALIGN_NONE, cl_size=0, stage=(64, 4, 2), max_threads=512, **dppw),
+         (140, 21, 13), (64, 4, 2)),
+    ]
+
+
 def smoke_desc():
     """3D Star Variable r=4, coarsening 1x4x16, DPP x exchange (the smoke run's kernel)."""
     return cfg2_desc(1, 4, 16, shape="dpp", max_threads=256)
@@ -237,6 +267,7 @@ def default_build_set():
     descs += [cfg4_desc(dt, "plain", batched=True, max_threads=1024).replace(remainder=True) for dt in (F64, F32C)]
     descs += [cfg4_desc(dt, "dppw", batched=True).replace(remainder=True) for dt in (F64, F32C)]
     descs += [c[1] for c in store_hazard_cases()]
+    descs += [d for d, _, _ in staged_cases()]
     from . import tuned
     descs += tuned.build_set()
     return list(dict.fromkeys(descs))

@@ -82,8 +82,14 @@ class StencilDesc:
     #                          adds it to every output that uses it; "offset" walks the mask in
     #                          summation order, every output per offset (weights used in order,
     #                          each loaded point kept until its last use)
+    stage: tuple = (0, 0, 0)  # STREAM 3D: LDS halo staging for the block shape (bx, by, bz) the
+    #                          kernel is then launched with: each z-slice of the block loads the
+    #                          rows of every plane once, with the y halo, into LDS (two buffers),
+    #                          and each thread reads its footprint rows from there instead of
+    #                          loading the y-arm rows its neighbours also load; (0, 0, 0): off
 
     def __post_init__(self):
+        object.__setattr__(self, "stage", tuple(self.stage))
         if self.dimension not in (2, 3):
             raise ValueError("dimension must be 2 or 3")
         if self.neighbourhood not in (BOX, STAR):
@@ -120,6 +126,20 @@ class StencilDesc:
             raise ValueError("unknown code variant")
         if self.variant != TEMPORAL and self.tsteps != 1:
             raise ValueError("tsteps > 1 needs the temporal variant")
+        if len(self.stage) != 3 or any(not isinstance(v, int) or v < 0 for v in self.stage):
+            raise ValueError("stage is a block shape (bx, by, bz), or (0, 0, 0)")
+        if self.staged:
+            bx, by, bz = self.stage
+            if (self.variant != STREAM or self.dimension != 3 or self.xext != "dppw" or self.remainder
+                    or self.batched or self.prefetch):
+                raise ValueError("LDS staging is an option of the 3D wave-aligned DPP STREAM shape "
+                                 "(without remainder handling, batching or register prefetch)")
+            if bx % 64 or not 64 <= bx <= 1024 or not 1 <= by <= 31 or not 1 <= bz <= 31:
+                raise ValueError("staged block: bx a multiple of 64 up to 1024, by and bz in [1, 31]")
+            if bx * by * bz > self.max_threads:
+                raise ValueError("staged block has more threads than the launch bound")
+            if self.stage_lds_bytes > 163840:
+                raise ValueError(f"staged block needs {self.stage_lds_bytes} bytes of LDS (gfx950: at most 160 KiB)")
         if self.variant == TEMPORAL:
             if not 1 <= self.tsteps <= 16:
                 raise ValueError("tsteps must be in [1, 16]")
@@ -173,6 +193,20 @@ class StencilDesc:
         return sum((2 * r + 1) * (tx + 2 * (k - s) * r) * (ty + 2 * (k - s) * r) for s in range(k)) * es
 
     @property
+    def staged(self) -> bool:
+        return tuple(self.stage) != (0, 0, 0)
+
+    @property
+    def stage_lds_bytes(self) -> int:
+        """STREAM with LDS staging: two buffers per z-slice of the block, each the
+        block's rows plus 2r halo rows by its columns."""
+        if not self.staged:
+            return 0
+        bx, by, bz = self.stage
+        es = 4 if self.dtype == F32C else 8
+        return bz * 2 * (by * self.yc + 2 * self.range) * bx * self.xc * es
+
+    @property
     def coarsening(self):
         return (self.xc, self.yc, self.zc)
 
@@ -199,8 +233,9 @@ class StencilDesc:
                 self.coeff, self.range, self.xc, self.yc, self.zc, self.alignment,
                 self.cl_size, self.dtype, self.variant, self.n, self.max_threads,
                 {"load": 0, "dpp": 1, "dppw": 2}[self.xext], int(self.xcd_remap),
-                int(self.nt_store) | int(self.nt_load) << 1 | int(self.remainder) << 2
-                | (self.tsteps if self.variant == TEMPORAL else 0) << 8,
+                int(self.nt_store) | int(self.nt_load) << 1 | int(self.remainder) << 2 | int(self.staged) << 3
+                | (self.tsteps if self.variant == TEMPORAL else 0) << 8
+                | (self.stage[0] // 64) << 16 | self.stage[1] << 21 | self.stage[2] << 26,
                 self.comp_group, self.prefetch, self.waves_per_eu, int(self.zfirst), int(self.batched)]
 
     def replace(self, **kw) -> "StencilDesc":
@@ -209,7 +244,8 @@ class StencilDesc:
 
 # knobs added after kernel names were first recorded (measured corpora refer to
 # kernels by name): left out of the name's hash while at their default
-_LATER_FIELDS = {"nt_load": False, "order": "point", "remainder": False, "edge_pairs": False, "tsteps": 1}
+_LATER_FIELDS = {"nt_load": False, "order": "point", "remainder": False, "edge_pairs": False, "tsteps": 1,
+                 "stage": (0, 0, 0)}
 
 
 def mask_index(desc: StencilDesc, dx: int, dy: int, dz: int = 0):

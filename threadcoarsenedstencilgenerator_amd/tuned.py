@@ -12,21 +12,34 @@ from __future__ import annotations
 from . import manifest
 from .spec import ALIGN_NONE, F32C, F64, ROW, STAR, STREAM, TEMPORAL, VARIABLE, StencilDesc
 
-# 3D Star Variable r=4, 512^3 interior, FP64, Write alignment at 16 points.  Blocks of two
-# 128-plane z-chunks (128x2x2, a 512-thread launch bound at the same 2 waves per SIMD) with the
-# wave-edge columns of two rows per load: 2.5% over the 128x2x1 block with one row per load in an
-# interleaved timing (300.3 against 293.1 GLup/s); each change alone gave about 1%.
+# 3D Star Variable r=4, 512^3 interior, FP64, Write alignment at 16 points.  LDS halo staging
+# for blocks of 128x4 threads (a 256x16-point tile per 128-plane z chunk, 98 KiB of LDS): each
+# plane's 24 rows are loaded once per block instead of 12 rows per thread for its 4 output rows,
+# 15.8% faster than the same coarsening loading its rows itself (346.9 against 299.5 GLup/s in an
+# interleaved timing; blocks of 128x2 threads: 322, 256x2: 325, 64x8: 310; 64-plane chunks 318;
+# the wave-edge columns one row per load 340; without nontemporal stores 329; without the XCD
+# remap 324).  The wave-edge columns of two rows per load, x-fastest tile order.
 CFG2_F64 = dict(desc=manifest.cfg2_desc(2, 4, 128, shape="dppw", max_threads=512).replace(
-    prefetch=0, zfirst=False, edge_pairs=True), block=(128, 2, 2))
-# the same workload in FP32 storage with compensated accumulation
-# FP32 rows are aligned to 32 points = 128 bytes, the line size the FP64 grid's 16 points
-# make (16 FP32 points, 64 bytes, leaves each row end's line written by two tiles: 5.6%
-# more write traffic); the first group sums the 21 terms up to the output's own plane,
-# the second the 4 terms of the planes after it, and TwoSum joins them.  The edge lanes fetch
-# the extra columns of two adjacent rows per load (edge_pairs: 16% fewer buffer loads, 3.7%
-# faster; the FP64 kernel gains nothing from it).
-CFG2_F32C = dict(desc=manifest.cfg2_desc(2, 8, 64, shape="dppw", max_threads=256, dtype=F32C).replace(
+    prefetch=0, zfirst=False, edge_pairs=True, stage=(128, 4, 1)), block=(128, 4, 1))
+# the same workload in FP32 storage with compensated accumulation: coarsening 4x4x64 with LDS
+# staging for blocks of 64x4 threads (a 256x16-point tile, 48 KiB of LDS; 615.6 GLup/s against
+# 563.2 for the unstaged 2x8x64 kernel; 4x4x64 staged for 64x8 threads: 592.5, 64x2: 590.8;
+# 2x4x64 staged for 128x4: 591.0; 2x4x128 kernels: 482-532).  FP32 rows are aligned to 32 points =
+# 128 bytes, the line size the FP64 grid's 16 points make; the first group sums the 21 terms up to
+# the output's own plane, the second the 4 terms of the planes after it, and TwoSum joins them.
+CFG2_F32C = dict(desc=manifest.cfg2_desc(4, 4, 64, shape="dppw", max_threads=256, dtype=F32C).replace(
+    prefetch=0, comp_group=21, zfirst=False, cl_size=32, edge_pairs=True, stage=(64, 4, 1)), block=(64, 4, 1))
+# the kernels that load their own rows (no LDS staging), fastest before staging: remainder handling
+# (a 511^3 interior) runs these, as staging covers whole coarsening tiles only
+CFG2_F64_UNSTAGED = dict(desc=CFG2_F64["desc"].replace(stage=(0, 0, 0)), block=(128, 2, 2))
+CFG2_F32C_UNSTAGED = dict(desc=manifest.cfg2_desc(2, 8, 64, shape="dppw", max_threads=256, dtype=F32C).replace(
     prefetch=0, comp_group=21, zfirst=False, cl_size=32, edge_pairs=True), block=(128, 2, 1))
+
+
+def remainder_cfg(dtype: int = F64):
+    """The 512^3 workload's kernel generated with remainder handling."""
+    c = CFG2_F32C_UNSTAGED if dtype == F32C else CFG2_F64_UNSTAGED
+    return dict(desc=c["desc"].replace(remainder=True), block=c["block"])
 
 
 # 2D Box Variable r=8 (289 weights), 16384x8192 interior, Read alignment at 16 points
@@ -86,6 +99,7 @@ def best(workload: str, dtype: int = F64):
 
 def build_set():
     return [CFG2_F64["desc"], CFG2_F32C["desc"], BOX2D_R8_F64["desc"], BOX2D_R8_F32C["desc"],
-            CFG2_F64["desc"].replace(remainder=True), CFG2_F32C["desc"].replace(remainder=True),
+            CFG2_F64_UNSTAGED["desc"], CFG2_F32C_UNSTAGED["desc"],
+            remainder_cfg(F64)["desc"], remainder_cfg(F32C)["desc"],
             BOX2D_R1_TEMPORAL["desc"], BOX2D_R1_ONE_STEP["desc"], BOX2D_R1_TEMPORAL_F32C["desc"],
             STAR3D_R1_TEMPORAL["desc"], STAR3D_R1_ONE_STEP["desc"]]

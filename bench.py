@@ -172,7 +172,17 @@ def bench_batched(args):
         del grids
         k.close()
         torch.cuda.empty_cache()
-    line = {"metric": "GLup/s, 64 uneven 3D Box Jacobi r=2 grids, 100 steps", "value": out["f64"]["glups"]["persistent_graph"],
+    # HBM roofline of the FP64 persistent + graph mode: one read and one write of every interior
+    # point per step, against the step rate; traffic from the counters where profiles/ has them
+    glups = out["f64"]["glups"]["persistent_graph"]
+    d64 = manifest.cfg4_desc(shape="plain", batched=True, dtype=F64, max_threads=1024)
+    tr = _traffic(d64.key(), block)
+    alg = 16.0 * out["f64"]["points"]
+    roof = {"bound": "hbm", "achieved": round(16.0 * glups, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(16.0 * glups / HBM_PEAK_GBS, 4),
+            "traffic": (round(tr["hbm_bytes_per_launch"] / tr["algorithmic_bytes_per_launch"] * alg) if tr else None),
+            "algorithmic_bytes_per_launch": int(alg)}
+    line = {"metric": "GLup/s, 64 uneven 3D Box Jacobi r=2 grids, 100 steps", "value": glups, "roofline": roof,
             "unit": "GLup/s", "n_gpus": 1, "steps": steps, "higher_is_better": True, "dtype": "f64",
             "data": "synthetic", "config": {"workload": "64 grids, interior axes log-uniform in [32,256] rounded "
                                             "to coarsening 2x2x4, U[0,1) values", "block": list(block),

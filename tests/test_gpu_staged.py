@@ -35,3 +35,13 @@ def test_staged_kernel_runs_its_block_only():
         assert e.value.status == 4   # TCSG_ERR_BLOCK_SHAPE
     k.update(g, block, rt.grid_blocks(g, block), grids.ptr(0), grids.ptr(1))
     del grids
+
+
+def test_staged_one_step_r1_workload_bitexact():
+    """The one-step baseline of the 3D temporal workload (3D Star r=1 on 512^3, LDS
+    staging for blocks of 256x2 threads): 2 steps bit for bit against the oracle."""
+    from threadcoarsenedstencilgenerator_amd import tuned
+    cfg = tuned.STAR3D_R1_ONE_STEP
+    print(f"staged one-step {cfg['desc'].key()} size {tuned.STAR3D_R1_GRID}", flush=True)
+    got, ref, _, _ = run_case(cfg["desc"], tuned.STAR3D_R1_GRID, steps=2, block=tuple(cfg["block"]), mode=2)
+    assert_bitwise(got, ref, cfg["desc"].key())

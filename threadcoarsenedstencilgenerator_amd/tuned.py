@@ -77,16 +77,18 @@ BOX2D_R1_ONE_STEP = dict(desc=manifest.cfg1_desc("stream").replace(xc=2, yc=4, m
 # 3D temporal kernel marches each 64x16-point tile through 128 planes, 2 steps per launch, with a ring
 # of 3 planes per step in LDS (61 KB, 512 threads).  Sweep over 1-4 steps per launch, tiles 64x8 to
 # 128x16 and 64x32, 32-128 planes, 256-1024 threads: 2 steps on 64x16x128 was fastest.  The one-step
-# baseline is the fastest one-step shape measured on the same grid (STREAM 2x4x128, x-neighbours by
-# loads; the wave-aligned DPP form ran within 1%).
+# baseline is the fastest one-step shape measured on the same grid: STREAM 2x4x128 with LDS halo
+# staging for blocks of 256x2 threads (323.1 GLup/s; staged for 128x4: 318.8, 128x2: 307.2; the
+# same kernel loading its own rows: 301.9; x-neighbours by loads: 298.1).
 STAR3D_R1_GRID = (514, 514, 514)
 STAR3D_R1_TEMPORAL = dict(desc=StencilDesc(dimension=3, neighbourhood=STAR, orientation=ROW, coeff=VARIABLE, range=1,
                                            xc=1, yc=2, zc=128, alignment=ALIGN_NONE, cl_size=0, variant=TEMPORAL,
                                            tsteps=2, max_threads=512, remainder=True), block=(64, 8, 1))
 STAR3D_R1_ONE_STEP = dict(desc=StencilDesc(dimension=3, neighbourhood=STAR, orientation=ROW, coeff=VARIABLE, range=1,
                                            xc=2, yc=4, zc=128, alignment=ALIGN_NONE, cl_size=0, variant=STREAM,
-                                           xext="load", xcd_remap=True, nt_store=True, max_threads=256),
-                          block=(128, 2, 1))
+                                           xext="dppw", xcd_remap=True, nt_store=True, prefetch=0, edge_pairs=True,
+                                           max_threads=512, stage=(256, 2, 1)),
+                          block=(256, 2, 1))
 
 
 def best(workload: str, dtype: int = F64):

@@ -268,6 +268,7 @@ def default_build_set():
     descs += [cfg4_desc(dt, "dppw", batched=True).replace(remainder=True) for dt in (F64, F32C)]
     descs += [c[1] for c in store_hazard_cases()]
     descs += [d for d, _, _ in staged_cases()]
-    from . import tuned
+    from . import tuned, tuner
     descs += tuned.build_set()
+    descs += [d for d, _ in tuner.live_configs()]   # the tuner's live check
     return list(dict.fromkeys(descs))

@@ -20,13 +20,19 @@ random launchable block shapes of each of 250 configurations (about 3800
 samples; its first 125 configurations, about 1900 samples, form the smaller
 training set the evaluation also reports); the held-out set times every
 launchable shape of 200 further configurations whose kernels the training
-set does not contain.
+set does not contain.  That pair is the evaluation protocol (train on the
+first set, score on the second); the committed model that ``tune`` uses is
+trained on both sets (about 50,800 samples) and carries the model-free
+ranking of their block shapes, which gets one of the k measurements.  The
+live check (``live``) sweeps 40 further configurations on the GPU it runs on
+and scores that model's picks against those sweeps.
 
     python -m threadcoarsenedstencilgenerator_amd.tuner build               # compile both sets' kernels
     python -m threadcoarsenedstencilgenerator_amd.tuner collect train       # measure (GPU), resumable
     python -m threadcoarsenedstencilgenerator_amd.tuner collect heldout
     python -m threadcoarsenedstencilgenerator_amd.tuner evaluate --json R   # top-k regret on held-out (GPU)
-    python -m threadcoarsenedstencilgenerator_amd.tuner fit                 # train the committed model (GPU)
+    python -m threadcoarsenedstencilgenerator_amd.tuner fit --train T H     # train the committed model (GPU)
+    python -m threadcoarsenedstencilgenerator_amd.tuner live --json R       # sweep 40 fresh configurations now (GPU)
 """
 from __future__ import annotations
 
@@ -151,6 +157,9 @@ MAX_TERMS = 4096          # outputs per thread x stencil size of a sampled kerne
 TRAIN_CONFIGS, HELDOUT_CONFIGS = 250, 200
 TRAIN_SHAPES_PER_CONFIG = 16
 TRAIN_SEED, HELDOUT_SEED = 11, 12
+# live check: further configurations, none of whose kernels either corpus contains, swept and
+# ranked on the GPU the check runs on (nothing about them was measured beforehand)
+LIVE_CONFIGS, LIVE_SEED = 40, 20261205
 
 
 def _desc(dim, nb, coeff, r, co, shape, mt, align=ALIGN_NONE, cl=0, dtype=F64, orient=ROW):
@@ -227,8 +236,14 @@ def heldout_configs():
     return sample_configs(HELDOUT_CONFIGS, HELDOUT_SEED, exclude={d.key() for d, _ in train_configs()})
 
 
+def live_configs():
+    """Configurations of the live check: kernels in neither the training nor the held-out set."""
+    return sample_configs(LIVE_CONFIGS, LIVE_SEED,
+                          exclude={d.key() for d, _ in train_configs() + heldout_configs()})
+
+
 def config_set(name: str):
-    return {"train": train_configs, "heldout": heldout_configs}[name]()
+    return {"train": train_configs, "heldout": heldout_configs, "live": live_configs}[name]()
 
 
 # -------------------------------------------------------------- measurements
@@ -366,6 +381,10 @@ class CostModel:
         self.mean = np.zeros(in_dim)
         self.std = np.ones(in_dim)
         self.y_mean = 0.0
+        # the model-free ranking of the corpora the model was trained on (shape_scores) and how
+        # many of tune()'s k measurements go to it (mixed_order)
+        self.fixed = {}
+        self.mix = 0
 
     def close(self):
         if getattr(self, "_h", None):
@@ -482,8 +501,11 @@ class CostModel:
         return self.forward_raw(self._norm(X)).astype(np.float64) + self.y_mean
 
     def save(self, path: str):
+        shapes = sorted(self.fixed)
         np.savez(path, params=self.params(), mean=self.mean, std=self.std, y_mean=np.array(self.y_mean),
-                 hidden=np.array(self.hidden), features=np.array(FEATURES))
+                 hidden=np.array(self.hidden), features=np.array(FEATURES), mix=np.array(self.mix),
+                 fixed_shapes=np.array(shapes, dtype=np.int64).reshape(-1, 3),
+                 fixed_scores=np.array([self.fixed[t] for t in shapes], dtype=np.float64))
 
     @classmethod
     def load(cls, path: str = DEFAULT_MODEL):
@@ -493,6 +515,9 @@ class CostModel:
         m = cls(hidden=int(z["hidden"]))
         m.set_params(z["params"])
         m.mean, m.std, m.y_mean = z["mean"], z["std"], float(z["y_mean"])
+        if "mix" in z.files:
+            m.mix = int(z["mix"])
+            m.fixed = {tuple(int(v) for v in t): float(sc) for t, sc in zip(z["fixed_shapes"], z["fixed_scores"])}
         return m
 
 
@@ -542,8 +567,23 @@ def shape_scores(corpus: Corpus):
     return {s: score[s] / cnt[s] for s in score}
 
 
+def mixed_order(pred_order, fixed_order, k: int, mix: int):
+    """The shapes to measure: the model's first k - mix, then the fixed ranking's first `mix`
+    not already chosen, then the rest in the model's order."""
+    head = list(pred_order[:k - mix])
+    seen = set(head)
+    for i in fixed_order:
+        if len(head) >= k:
+            break
+        if i not in seen:
+            head.append(i)
+            seen.add(i)
+    head += [i for i in pred_order if i not in seen]
+    return np.array(head)
+
+
 def evaluate(train: Corpus, heldout: Corpus, k: int = 8, seed: int = 0, epochs: int = 60, hidden: int = 128,
-             model: "CostModel | None" = None, log=None):
+             model: "CostModel | None" = None, log=None, mix: int = 0, scores=None):
     """Train the cost model on `train` (or use `model`), then for every
     held-out (kernel, grid) group rank its candidates, 'measure' the top k
     (look them up in the group's exhaustive sweep) and record the regret
@@ -551,7 +591,9 @@ def evaluate(train: Corpus, heldout: Corpus, k: int = 8, seed: int = 0, epochs: 
     k measurements plus the model's inference cost.  Next to it: two
     model-free strategies with the same k, k shapes at random (expected
     regret) and the k shapes with the best mean relative throughput over
-    the training groups."""
+    the training groups.  mix > 0 gives that many of the k measurements to
+    the second ranking (mixed_order); scores replaces the model-free ranking
+    computed from `train`."""
     fit_s = 0.0
     own = model is None
     if own:
@@ -561,7 +603,7 @@ def evaluate(train: Corpus, heldout: Corpus, k: int = 8, seed: int = 0, epochs: 
         fit_s = time.perf_counter() - t0
         if log:
             log(f"trained on {len(train.rows)} rows from {len(train.groups)} groups in {fit_s:.1f}s")
-    scores = shape_scores(train)
+    scores = shape_scores(train) if scores is None else scores
     per_group = []
     spent = {"model": 0.0, "fixed": 0.0, "sweep": 0.0}
     for gk, idx in heldout.groups.items():
@@ -572,9 +614,11 @@ def evaluate(train: Corpus, heldout: Corpus, k: int = 8, seed: int = 0, epochs: 
         pred = model.predict(heldout.X[idx])
         order = np.argsort(-pred, kind="stable")
         infer_ms = (time.perf_counter() - t0) * 1e3
+        fixed = np.argsort(-np.array([scores.get(heldout.shape[i], 0.0) for i in idx]), kind="stable")
+        if mix:
+            order = mixed_order(order, fixed, k, mix)
         reg, frac = top_k_outcome(g, cost, order, k)
         frac += infer_ms / cost.sum()
-        fixed = np.argsort(-np.array([scores.get(heldout.shape[i], 0.0) for i in idx]), kind="stable")
         freg, ffrac = top_k_outcome(g, cost, fixed, k)
         spent["model"] += cost[order[:k]].sum() + infer_ms
         spent["fixed"] += cost[fixed[:k]].sum()
@@ -607,9 +651,40 @@ def evaluate(train: Corpus, heldout: Corpus, k: int = 8, seed: int = 0, epochs: 
                 per_group=per_group)
 
 
+def live_evaluate(configs=None, k: int = 8, model: "CostModel | None" = None, log=None, rows_out=None):
+    """The held-out evaluation re-measured where it runs: sweep every launchable block shape
+    of each configuration now (measure_sweep), rank the same candidates with the committed
+    model, and score the k shapes tune() would measure (the model's ranking mixed with the
+    model-free one it carries) against that sweep (regret, share of the sweep's GPU time),
+    next to k random shapes and the model-free ranking's k."""
+    configs = live_configs() if configs is None else configs
+    rows = []
+    t0 = time.perf_counter()
+    for j, (d, s) in enumerate(configs):
+        got = measure_sweep(d, s)
+        rows += got
+        if log:
+            best = max(got, key=lambda r: r["glups"])
+            log(f"[{j}] {d.key()} {s}: {len(got)} shapes, best {best['glups']:.1f} GLup/s "
+                f"at {best['bx']}x{best['by']}x{best['bz']}")
+    sweep_s = time.perf_counter() - t0
+    if rows_out:
+        write_rows(rows_out, rows)
+    own = model is None
+    model = CostModel.load() if own else model
+    rep = evaluate(Corpus([]), Corpus(rows), k=k, model=model, mix=model.mix, scores=model.fixed)
+    rep["mix"] = model.mix
+    if own:
+        model.close()
+    rep["wall_seconds_sweeps"] = sweep_s
+    return rep
+
+
 # ------------------------------------------------------------------- tuning
-def rank(kernel, size, model: "CostModel"):
-    """The kernel's launchable sweep shapes, best predicted first."""
+def rank(kernel, size, model: "CostModel", k: int = 8):
+    """The kernel's launchable sweep shapes in the order tune() measures them: best predicted
+    first, with `model.mix` of the first k taken from the model-free ranking the model
+    carries (mixed_order)."""
     desc = kernel.desc
     shapes = candidates(kernel, desc.dimension)
     occ = {}
@@ -620,7 +695,11 @@ def rank(kernel, size, model: "CostModel"):
             occ[t] = kernel.resources(t)
         X.append(features(desc, size, s, occ[t]))
     pred = model.predict(np.stack(X))
-    return [shapes[i] for i in np.argsort(-pred, kind="stable")]
+    order = np.argsort(-pred, kind="stable")
+    if model.mix and model.fixed:
+        fixed = np.argsort(-np.array([model.fixed.get(tuple(s), 0.0) for s in shapes]), kind="stable")
+        order = mixed_order(order, fixed, k, model.mix)
+    return [shapes[i] for i in order]
 
 
 def tune(kernel, size, k: int = 8, model: "CostModel | None" = None, d0=None, d1=None, steps: int = 2):
@@ -633,7 +712,7 @@ def tune(kernel, size, k: int = 8, model: "CostModel | None" = None, d0=None, d1
     from . import runtime as rt
     if model is None and os.path.exists(DEFAULT_MODEL):
         model = CostModel.load()
-    shapes = rank(kernel, size, model)[:k] if model is not None else candidates(kernel, kernel.desc.dimension)
+    shapes = rank(kernel, size, model, k)[:k] if model is not None else candidates(kernel, kernel.desc.dimension)
     g = kernel.geometry(size)
     grids = None
     if d0 is None:
@@ -685,10 +764,16 @@ def main(argv=None):
                                   "'Star Row Variable 4 2 4 128 Write 16 --xext=dppw'")
     t.add_argument("--grid", type=int, nargs="+", required=True, help="grid size incl. halo, x y [z]")
     t.add_argument("--k", type=int, default=8)
+    lv = sub.add_parser("live", help="sweep fresh configurations now and score the committed model's top k")
+    lv.add_argument("--json", default=None)
+    lv.add_argument("--k", type=int, default=8)
+    lv.add_argument("--rows", default=None, help="also write the sweeps' rows (csv.gz) here")
     f = sub.add_parser("fit")
-    f.add_argument("--train", default=TRAIN_DATA)
+    f.add_argument("--train", nargs="+", default=[TRAIN_DATA])
     f.add_argument("--model", default=DEFAULT_MODEL)
     f.add_argument("--epochs", type=int, default=60)
+    f.add_argument("--mix", type=int, default=1, help="of tune()'s k measurements, how many go to the "
+                                                      "model-free ranking of the training corpora")
     a = ap.parse_args(argv)
 
     def log(msg):
@@ -696,7 +781,16 @@ def main(argv=None):
 
     if a.cmd == "build":
         from . import build as B
-        B.build_kernels([d for d, _ in train_configs() + heldout_configs()], force=False)
+        B.build_kernels([d for d, _ in train_configs() + heldout_configs() + live_configs()], force=False)
+        return 0
+    if a.cmd == "live":
+        import torch
+        torch.cuda.set_device(0)
+        rep = live_evaluate(k=a.k, log=log, rows_out=a.rows)
+        log(json.dumps({kk: v for kk, v in rep.items() if kk != "per_group"}, indent=1))
+        if a.json:
+            with open(a.json, "w") as fh:
+                json.dump(rep, fh, indent=1)
         return 0
     if a.cmd == "collect":
         import torch
@@ -757,9 +851,10 @@ def main(argv=None):
     if a.cmd == "fit":
         import torch
         torch.cuda.set_device(0)
-        cp = Corpus(read_rows(a.train))
+        cp = Corpus([r for p in a.train for r in read_rows(p)])
         m = CostModel()
         losses = m.fit(cp.X, cp.y, epochs=a.epochs)
+        m.fixed, m.mix = shape_scores(cp), a.mix
         m.save(a.model)
         log(f"trained on {len(cp.rows)} rows: loss {losses[0]:.4f} -> {losses[-1]:.4f}; saved {a.model}")
         return 0

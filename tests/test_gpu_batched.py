@@ -119,16 +119,19 @@ def test_graph_replays_with_work_between(dtype):
 @pytest.mark.parametrize("dtype", [F64, F32C])
 def test_benchmark_batch_matches_oracle(dtype):
     """The batch bench.py times: 64 grids with interior axes log-uniform in [32, 256]
-    (rounded to the 2x2x4 coarsening), block 32x8x2, the occupancy-sized persistent
-    grid (one range of the tile table per XCD, workgroups draining other XCDs'
-    ranges, the last workgroup clearing the queue), 3 steps through the captured
-    graph, replayed twice from the restored state: every grid equals the oracle."""
+    (rounded to the 2x2x4 coarsening), the benchmarked kernel (z chunks of 8 planes with
+    remainder handling) and block, the occupancy-sized persistent grid (one range of the
+    tile table per XCD, workgroups draining other XCDs' ranges, the last workgroup clearing
+    the queue), 3 steps through the captured graph, replayed twice from the restored state:
+    every grid equals the oracle."""
     import torch
 
     from threadcoarsenedstencilgenerator_amd import runtime as rt
-    d = manifest.cfg4_desc(shape="plain", batched=True, dtype=dtype, max_threads=1024)
+    from threadcoarsenedstencilgenerator_amd import tuned
+    cfg = tuned.best("batched", dtype)
+    d, block = cfg["desc"], tuple(cfg["block"])
     k = rt.Kernel(d)
-    sizes = rt.batch_sizes(64, 1, 32, 256, d.coarsening, d.range)
+    sizes = rt.batch_sizes(64, 1, 32, 256, tuned.BATCH_ROUNDING, d.range)
     geoms, grids = [], []
     for i, size in enumerate(sizes):
         g = k.geometry(size)
@@ -139,14 +142,14 @@ def test_benchmark_batch_matches_oracle(dtype):
         grids.append(gr)
     torch.cuda.synchronize()
     init = [gr.host(0) for gr in grids]
-    b = rt.Batch(k, geoms, [gr.ptr(0) for gr in grids], [gr.ptr(1) for gr in grids], (32, 8, 2), workgroups=0)
+    b = rt.Batch(k, geoms, [gr.ptr(0) for gr in grids], [gr.ptr(1) for gr in grids], block, workgroups=0)
     pts, tiles, nwg = b.info()
     print(f"benchmark batch {d.key()}: {len(sizes)} grids, {pts} points, {tiles} tiles, {nwg} workgroups",
           flush=True)
     assert len(sizes) == 64 and nwg >= 256 and tiles > nwg
     steps = 3
     run = oracle.run_f32c if dtype == F32C else oracle.run_f64
-    refs = [run(d, oracle.Geometry(3, d.range, d.alignment, d.cl_size, size, d.coarsening), None, a.copy(),
+    refs = [run(d, oracle.Geometry(3, d.range, d.alignment, d.cl_size, size, tuned.BATCH_ROUNDING), None, a.copy(),
                 a.copy(), steps) for size, a in zip(sizes, init)]
     ui = np.uint32 if dtype == F32C else np.uint64
     for replay in range(2):

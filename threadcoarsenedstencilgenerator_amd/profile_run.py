@@ -10,6 +10,9 @@ counter pass measures both.
         (--workload box2d_r1_temporal: the temporal 2D Box r=1 kernel on 16384^2 instead;
          --workload all --meta FILE: every measured workload in one pass, see all_workloads)
 
+With --batched-sweep it instead times the batched workload's kernel candidates
+(coarsening and block shape, each checked bit for bit against the 2x2x4 kernel).
+
 With --ceiling it instead times the HBM stream probes (copy, read-only,
 write-only; plain and nontemporal; 1-16 loads of 16 bytes in flight per lane,
 256-1024 threads per workgroup; one pass and grid-stride launches) over
@@ -48,6 +51,73 @@ def ceiling(reps: int = 20):
                     print(json.dumps(line), flush=True)
     best = {m: max(x["tbs"] for x in out if x["probe"] == m) for m in ("copy", "read", "write")}
     print(json.dumps({"hbm_ceiling_tbs": best, "peak_tbs": 8.0}), flush=True)
+
+
+def batched_candidates(dtype):
+    """(name, desc, block) of the batched persistent kernels the coarsening and block sweep
+    times: the reference's code shape (PLAIN, launch bound 1024 unless named) with remainder
+    handling, so any coarsening runs the workload's 2x2x4-rounded grids untruncated."""
+    from . import manifest
+    base = manifest.cfg4_desc(shape="plain", batched=True, dtype=dtype, max_threads=1024)
+    out = [("2x2x4", base, (32, 8, 2))]
+    table = (((2, 2, 8), 1024, ((32, 8, 2), (32, 16, 1), (32, 4, 4), (32, 8, 1), (32, 8, 4), (16, 16, 2),
+                               (64, 8, 1), (64, 4, 2))),
+             ((2, 2, 12), 1024, ((32, 8, 2),)), ((2, 2, 16), 1024, ((32, 8, 2),)), ((2, 2, 6), 1024, ((32, 8, 2),)),
+             ((2, 1, 8), 1024, ((32, 16, 1),)), ((1, 2, 8), 1024, ((32, 8, 2),)), ((2, 4, 4), 1024, ((32, 8, 2),)),
+             ((4, 2, 4), 1024, ((16, 8, 2),)), ((4, 4, 4), 256, ((16, 8, 2),)), ((4, 4, 2), 1024, ((16, 8, 2),)),
+             ((4, 2, 8), 256, ((16, 8, 2),)))
+    for co, mt, blocks in table:
+        d = base.replace(xc=co[0], yc=co[1], zc=co[2], remainder=True, max_threads=mt)
+        for blk in blocks:
+            out.append((f"{co[0]}x{co[1]}x{co[2]}" + ("" if mt == 1024 else f"_mt{mt}"), d, blk))
+    return out
+
+
+def batched_sweep(dtype_name: str, steps: int = 20):
+    """The batched workload (64 grids, interiors log-uniform in [32, 256] rounded to 2x2x4)
+    through each candidate of batched_candidates: 2 steps from the seeded state compared bit
+    for bit with the 2x2x4 kernel's, then GLup/s over `steps` graph-replayed steps.  One JSON
+    line per candidate.  Kernels not in the tree are compiled first."""
+    import json
+    import time
+
+    import numpy as np
+    import torch
+
+    from . import build as B
+    from . import runtime as rt
+    from . import spec, tuned
+    torch.cuda.set_device(0)
+    dt = spec.F32C if dtype_name == "f32c" else spec.F64
+    ui = np.uint32 if dt == spec.F32C else np.uint64
+    stream = torch.cuda.current_stream()
+    ref = None
+    for name, d, block in batched_candidates(dt):
+        B.build_kernel(d, force=False)
+        k = rt.Kernel(d)
+        sizes = rt.batch_sizes(64, 1, 32, 256, tuned.BATCH_ROUNDING, d.range)
+        geoms = [k.geometry(sz) for sz in sizes]
+        grids = [_setup(k, g, rt.INIT_U01, 100 + i) for i, g in enumerate(geoms)]
+        bt = rt.Batch(k, geoms, [g.ptr(0) for g in grids], [g.ptr(1) for g in grids], block)
+        pts, tiles, nwg = bt.info()
+        fin = bt.run(2, graph=False, stream=stream)
+        torch.cuda.synchronize()
+        res = [g.host(fin) for g in grids]
+        same = ref is None or all(np.array_equal(a.view(ui), c.view(ui)) for a, c in zip(ref, res))
+        ref = res if ref is None else ref
+        bt.run(steps, graph=True, stream=stream)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        bt.run(steps, graph=True, stream=stream)
+        torch.cuda.synchronize()
+        glups = pts * steps / (time.perf_counter() - t0) / 1e9
+        print(json.dumps(dict(dtype=dtype_name, coarsening=name, block=list(block), kernel=d.key(),
+                              glups=round(glups, 2), tiles=tiles, workgroups=nwg, same_as_2x2x4=bool(same))),
+              flush=True)
+        bt.close()
+        del grids
+        k.close()
+        torch.cuda.empty_cache()
 
 
 def _setup(k, g, mode, seed):
@@ -115,12 +185,11 @@ def all_workloads(steps: int, meta: str):
         k.close()
         torch.cuda.empty_cache()
     # batched: 64 uneven 3D Box Jacobi r=2 grids, one persistent launch per step
-    d = manifest.cfg4_desc(shape="plain", batched=True, max_threads=1024)
+    d, block = tuned.BATCHED_F64["desc"], tuple(tuned.BATCHED_F64["block"])
     k = rt.Kernel(d)
-    sizes = rt.batch_sizes(64, 1, 32, 256, d.coarsening, d.range)
+    sizes = rt.batch_sizes(64, 1, 32, 256, tuned.BATCH_ROUNDING, d.range)
     geoms = [k.geometry(sz) for sz in sizes]
     gr = [_setup(k, g, rt.INIT_U01, 100 + i) for i, g in enumerate(geoms)]
-    block = (32, 8, 2)
     bt = rt.Batch(k, geoms, [x.ptr(0) for x in gr], [x.ptr(1) for x in gr], block)
     pts, tiles, nwg = bt.info()
     out.append(dict(name="batched_64_box3d_r2_f64", kernel=d.key(), symbol="tcsg_update_batched", kind="batched",
@@ -143,9 +212,13 @@ def main():
     ap.add_argument("--workload", default="star3d_r4_512", choices=("star3d_r4_512", "box2d_r1_temporal", "all"))
     ap.add_argument("--meta", default="profile_workloads.json", help="--workload all: where to write the "
                     "dispatch table")
+    ap.add_argument("--batched-sweep", action="store_true", help="time the batched kernel's coarsening and "
+                    "block candidates (--dtype f64|f32c) instead")
     args = ap.parse_args()
     if args.ceiling:
         return ceiling()
+    if args.batched_sweep:
+        return batched_sweep(args.dtype)
     if args.workload == "all":
         return all_workloads(args.steps, args.meta)
     import torch

@@ -45,10 +45,11 @@ PROBE_CO = os.path.join(KDIR, "buffer_probe.co")
 
 
 def build_probe() -> str:
-    """Code object of the buffer range-check probe (csrc/buffer_probe.hip)."""
+    """Code object of the buffer range-check probe the GPU tests load (a test fixture:
+    tests/fixtures/buffer_probe.hip)."""
     os.makedirs(KDIR, exist_ok=True)
-    _run([HIPCC, *HIP_FLAGS, "--genco", "--no-gpu-bundle-output", os.path.join(PKG, "csrc", "buffer_probe.hip"),
-          "-o", PROBE_CO])
+    src = os.path.join(os.path.dirname(PKG), "tests", "fixtures", "buffer_probe.hip")
+    _run([HIPCC, *HIP_FLAGS, "--genco", "--no-gpu-bundle-output", src, "-o", PROBE_CO])
     return PROBE_CO
 
 

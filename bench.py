@@ -280,38 +280,42 @@ def bench_box2d_r8(args):
 
 
 def bench_remainder(args):
-    """The 512^3 workload's kernel generated with remainder handling (the
-    coarsening and code shape of the production kernel, loading its own rows:
-    LDS staging covers whole coarsening tiles only), on a 511^3 interior (no
-    axis a multiple of its coarsening): every point updated, against the same
+    """The 512^3 workload's production kernel (LDS staging) generated with
+    remainder handling, on a 511^3 interior (no axis a multiple of its
+    coarsening or of its block's tile): every point updated, against the same
     kernel without the option, which truncates that grid as the reference does
     (z to a multiple of its 128-plane chunk), and against itself on the 512^3
-    grid (the cost of the store masks)."""
+    grid (the cost of the row-group stores); the remainder-handling kernel that
+    loads its own rows is timed on the 511^3 grid too."""
     import torch
 
     from threadcoarsenedstencilgenerator_amd import runtime as rt
     from threadcoarsenedstencilgenerator_amd import tuned
-    best = tuned.CFG2_F64_UNSTAGED
+    best = tuned.CFG2_F64
     block = tuple(best["block"])
+    unstaged = tuned.remainder_cfg(staged=False)
     kr, kt = rt.Kernel(tuned.remainder_cfg()["desc"]), rt.Kernel(best["desc"])
+    ku = rt.Kernel(unstaged["desc"])
     mask = rt.mask_seeded(MASK_SEED, best["desc"].n)
     stream = torch.cuda.current_stream()
     out = {}
-    for name, k, grid in (("remainder_511", kr, (519, 519, 519)), ("truncating_511", kt, (519, 519, 519)),
-                          ("remainder_512", kr, GRID), ("truncating_512", kt, GRID)):
+    for name, k, grid, blk in (("remainder_511", kr, (519, 519, 519), block),
+                               ("truncating_511", kt, (519, 519, 519), block),
+                               ("remainder_512", kr, GRID, block), ("truncating_512", kt, GRID, block),
+                               ("remainder_unstaged_511", ku, (519, 519, 519), tuple(unstaged["block"]))):
         k.set_mask(mask)
         g = k.geometry(grid)
         grids = rt.Grids(g, k.dtype)
         for i in range(2):
             k.init(g, grids.ptr(i), rt.INIT_NORMAL, INPUT_SEED)
-        blocks = rt.grid_blocks(g, block)
-        k.run(g, block, blocks, grids.ptr(0), grids.ptr(1), args.warmup, stream)
+        blocks = rt.grid_blocks(g, blk)
+        k.run(g, blk, blocks, grids.ptr(0), grids.ptr(1), args.warmup, stream)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        k.run(g, block, blocks, grids.ptr(0), grids.ptr(1), args.steps, stream)
+        k.run(g, blk, blocks, grids.ptr(0), grids.ptr(1), args.steps, stream)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
-        ms = k.time_steps(g, block, blocks, grids.ptr(0), grids.ptr(1), 0, args.steps, stream) / args.steps
+        ms = k.time_steps(g, blk, blocks, grids.ptr(0), grids.ptr(1), 0, args.steps, stream) / args.steps
         user = (grid[0] - 2 * g.range) * (grid[1] - 2 * g.range) * (grid[2] - 2 * g.range)
         out[name] = {"glups": round(g.interior_points * args.steps / dt / 1e9, 2), "kernel_ms": round(ms, 4),
                      "interior": list(g.elements.tup()), "coverage": round(g.interior_points / user, 4),

@@ -328,6 +328,51 @@ def test_emulated_staged_kernel_matches_oracle(case):
     assert diff.size == 0, f"{diff.size} points differ, first {diff[:5]}"
 
 
+STAGED_REMAINDER_CASES = manifest.staged_remainder_cases()
+
+
+def _staged_remainder_run(d, size, block, edit=None):
+    geo = spec.geometry(d, size)
+    og = oracle.Geometry(3, d.range, d.alignment, d.cl_size, size, (1, 1, 1))
+    a = np.zeros(og.n_points)
+    oracle.init_seeded(og, 1, 4, a, halo=0.375)
+    mask = oracle.mask_seeded(6, d.n)
+    dt = np.float32 if d.dtype == F32C else np.float64
+    a = a.astype(dt)
+    run = oracle.run_f32c if d.dtype == F32C else oracle.run_f64
+    ref = run(d, og, mask, a.copy(), a.copy(), 2)
+    nblk = tuple((geo.threads[i] + block[i] - 1) // block[i] for i in range(3))
+    got, bad = emulate.run(d, geo, mask.astype(dt), a.copy(), a.copy(), 2, block, nblk, edit=edit)
+    ui = np.uint32 if dt == np.float32 else np.uint64
+    return np.flatnonzero(got.view(ui) != ref.view(ui)), bad, nblk, geo
+
+
+@pytest.mark.parametrize("case", STAGED_REMAINDER_CASES, ids=[c[0].key() for c in STAGED_REMAINDER_CASES])
+def test_emulated_staged_remainder_kernel_matches_oracle(case):
+    """LDS-staging kernels with remainder handling on grids whose interiors are multiples of
+    neither the coarsening nor the block's tile: every interior point equals the untruncated
+    oracle over 2 steps, every LDS access ordered by the barriers."""
+    d, size, block = case
+    diff, bad, nblk, geo = _staged_remainder_run(d, size, block)
+    assert any(geo.elements[i] % (block[i] * d.coarsening[i]) for i in (1, 2))
+    assert bad == 0, f"{bad} bad accesses"
+    assert diff.size == 0, f"{diff.size} points differ, first {diff[:5]}"
+
+
+def test_emulated_staged_remainder_needs_the_block_shift():
+    """Without the shift of the last block of rows (its tile then runs past the interior),
+    points differ from the oracle."""
+    d, size, block = STAGED_REMAINDER_CASES[1]
+    ty = block[1] * d.yc
+
+    def edit(text):
+        old = f"tcsg_min(byi * {ty}u, ydim - {2 * d.range + ty}u)"
+        assert old in text
+        return text.replace(old, f"(byi * {ty}u)")
+    diff, bad, _, _ = _staged_remainder_run(d, size, block, edit=("noshift", edit))
+    assert diff.size > 0
+
+
 def _drop_barrier(k):
     def edit(text):
         lines = text.split("\n")
@@ -363,7 +408,7 @@ def test_stage_descriptor_limits():
             base.replace(**bad)
     with pytest.raises(ValueError, match="LDS"):
         base.replace(stage=(256, 2, 1), yc=8)                 # 2 * 24 * 512 * 8 B > 160 KiB
-    for other in (dict(xext="dpp"), dict(remainder=True), dict(prefetch=1), dict(batched=True)):
+    for other in (dict(xext="dpp"), dict(prefetch=1), dict(batched=True)):
         with pytest.raises(ValueError, match="staging"):
             base.replace(stage=(64, 2, 1), **other)
     with pytest.raises(ValueError, match="staging"):

@@ -52,12 +52,13 @@ def test_production_kernel_bitexact_512(which, steps):
     assert bad.size == 0, f"{bad.size} points differ from the oracle"
 
 
-@pytest.mark.parametrize("which", ["f64", "f32c"])
+@pytest.mark.parametrize("which", ["f64", "f32c", "f64_unstaged", "f32c_unstaged"])
 def test_remainder_kernel_bitexact_511(which):
-    """The workload's kernels generated with remainder handling (loading their own rows) on a
-    511^3 interior (no axis a multiple of the 2x4x128 / 2x8x64 coarsening): every
-    point updated, bit for bit against the untruncated oracle over 3 steps."""
-    cfg = tuned.remainder_cfg(F64 if which == "f64" else F32C)
+    """The workload's kernels generated with remainder handling (the staged production
+    kernels, and those loading their own rows) on a 511^3 interior (no axis a multiple of
+    the coarsening or of the staged block's tile): every point updated, bit for bit against
+    the untruncated oracle over 3 steps."""
+    cfg = tuned.remainder_cfg(F32C if which.startswith("f32c") else F64, staged=not which.endswith("unstaged"))
     d = cfg["desc"]
     grid = (519, 519, 519)
     print(f"full-size remainder {which} {d.key()} grid {grid}", flush=True)

@@ -21,6 +21,38 @@ def test_staged_kernel_bitexact(case):
     assert_bitwise(got, ref, f"{d.key()} block {block}")
 
 
+REMAINDER_CASES = manifest.staged_remainder_cases()
+
+
+@pytest.mark.parametrize("case", REMAINDER_CASES, ids=[c[0].key() for c in REMAINDER_CASES])
+def test_staged_remainder_kernel_bitexact(case):
+    """Remainder handling with LDS staging (the last block of rows and of planes starts
+    early, the last lanes in x drop their columns past the interior) on grids that are
+    multiples of neither the coarsening nor the tile: every point against the untruncated
+    oracle over 3 steps."""
+    d, size, block = case
+    print(f"staged remainder {d.key()} size {size} block {block}", flush=True)
+    got, ref, _, _ = run_case(d, size, steps=3, block=block, mode=2)
+    assert_bitwise(got, ref, f"{d.key()} block {block}")
+
+
+def test_staged_remainder_refuses_an_interior_below_one_tile():
+    """The runtime refuses a grid whose interior is smaller than one block's tile in y (the
+    last block of rows could not start early enough), as TCSG_ERR_ARG, before any launch."""
+    from threadcoarsenedstencilgenerator_amd import runtime as rt
+    d, size, block = REMAINDER_CASES[1]
+    small = (size[0], 2 * d.range + block[1] * d.yc - 1, size[2])
+    print(f"staged remainder {d.key()} refuses size {small}", flush=True)
+    k = rt.Kernel(d)
+    g = k.geometry(small)
+    k.set_mask(rt.mask_seeded(3, d.n))
+    grids = rt.Grids(g, k.dtype)
+    with pytest.raises(rt.TcsgError) as e:
+        k.update(g, block, rt.grid_blocks(g, block), grids.ptr(0), grids.ptr(1))
+    assert e.value.status == 1   # TCSG_ERR_ARG
+    del grids
+
+
 def test_staged_kernel_runs_its_block_only():
     from threadcoarsenedstencilgenerator_amd import runtime as rt
     d, size, block = CASES[1]

@@ -334,7 +334,7 @@ int tcsg_kernel_load(tcsg_kernel **out, const char *path, const tcsg_desc *expec
     if (k->tsteps < 1 || (d.variant == TCSG_TEMPORAL && q[15] % 64)) return fail(TCSG_ERR_MISMATCH);
     if (q[18] & 8u) {
         k->stage = tcsg_dim3{((q[18] >> 16) & 31u) * 64u, (q[18] >> 21) & 31u, (q[18] >> 26) & 31u};
-        if (d.variant != TCSG_STREAM || d.dimension != 3 || k->xext_mode != 2 || k->remainder || !k->stage.x ||
+        if (d.variant != TCSG_STREAM || d.dimension != 3 || k->xext_mode != 2 || !k->stage.x ||
             !k->stage.y || !k->stage.z)
             return fail(TCSG_ERR_MISMATCH);
     }
@@ -576,6 +576,12 @@ static int precheck(tcsg_kernel *k, const tcsg_geometry *g, tcsg_dim3 block, tcs
         rc = stream_limits(k->desc, g, window_extent(k->desc, block), load_rows(k, block), nullptr);
         if (rc) return rc;
     }
+    // an LDS-staging kernel with remainder handling starts its last block of rows (planes)
+    // early enough to end at the interior's end: the interior must hold one block's tile
+    if (k->stage.x && k->remainder &&
+        ((uint64_t)g->elements.y < (uint64_t)k->stage.y * k->desc.yc ||
+         (k->desc.dimension == 3 && (uint64_t)g->elements.z < (uint64_t)k->stage.z * k->desc.zc)))
+        return TCSG_ERR_ARG;
     if (!blocks.x || !blocks.y || !blocks.z) return TCSG_ERR_ARG;
     if ((uint64_t)blocks.x * block.x < g->threads.x || (uint64_t)blocks.y * block.y < g->threads.y ||
         (uint64_t)blocks.z * block.z < g->threads.z)

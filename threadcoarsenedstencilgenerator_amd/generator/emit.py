@@ -553,7 +553,13 @@ class _Emitter:
         w("  // inrow: the thread's row/plane is inside the interior; such lanes load (and")
         w("  // can feed x-neighbours to other lanes) even past the interior's x end, where")
         w("  // they read the halo.  valid: the thread owns interior points and stores them.")
-        w("  const lbool inrow = gy < thy && gz < thz;")
+        # LDS staging with remainder handling shifts whole blocks (below), so every thread row
+        # and plane of a launched block is inside the interior
+        sremain = d.staged and d.remainder
+        if sremain:
+            w("  const lbool inrow = true;")
+        else:
+            w("  const lbool inrow = gy < thy && gz < thz;")
         w("  const lbool valid = inrow && gx < thx;")
         w(f"  const u32 x0 = gx * {XC}u + {R + LO}u;")
         if d.remainder and (self.YC > 1 or (self.three and self.ZC > 1)):
@@ -562,12 +568,25 @@ class _Emitter:
             w("  // shares with its neighbour get the same value from both (same inputs, same")
             w("  // summation order).  The runtime requires interiors of at least one coarsening")
             w("  // in y and z.")
-        if d.remainder and self.YC > 1:
+        if sremain:
+            BX_, BY_, BZ_ = d.stage
+            TYs = BY_ * self.YC
+            w("  // remainder handling with LDS staging: the block's tile (not one thread) starts early")
+            w("  // enough to end at the interior's end; the rows it shares with the block before it get")
+            w("  // the same bits from both.  The runtime requires an interior of at least one tile in y")
+            w("  // and z.")
+            w(f"  const u32 yb = tcsg_min(byi * {TYs}u, ydim - {2 * R + TYs}u);  // the block's first footprint row")
+            w(f"  const u32 y0 = yb + threadIdx.y * {self.YC}u;            // first row of the footprint")
+        elif d.remainder and self.YC > 1:
             w(f"  const u32 y0 = tcsg_min(gy * {self.YC}u, ydim - {2 * R + self.YC}u);  // first row of the footprint")
         else:
             w(f"  const u32 y0 = gy * {self.YC}u;            // first row of the footprint")
         if not self.three:
             w("  const u32 z0 = 0u;")
+        elif sremain:
+            w(f"  const u32 wb = tcsg_min(bzi * {BZ_ * self.ZC}u, zdim - {2 * R + BZ_ * self.ZC}u);"
+              f"  // the block's first footprint plane")
+            w(f"  const u32 z0 = wb + threadIdx.z * {self.ZC}u;            // first plane of the footprint")
         elif d.remainder and self.ZC > 1:
             w(f"  const u32 z0 = tcsg_min(gz * {self.ZC}u, zdim - {2 * R + self.ZC}u);  // first plane of the footprint")
         else:
@@ -579,8 +598,9 @@ class _Emitter:
         # (its planes or rows plus the footprint) stays below the out-of-range base
         if self.three:
             zcl = f", zdim - {2 * R + self.ZC}u" if d.remainder and self.ZC > 1 else ""
-            w(f"  const u32 wb = {'tcsg_min(' if zcl else '('}bzi * blockDim.z * {self.ZC}u{zcl});"
-              f"  // the block's first footprint plane")
+            if not sremain:
+                w(f"  const u32 wb = {'tcsg_min(' if zcl else '('}bzi * blockDim.z * {self.ZC}u{zcl});"
+                  f"  // the block's first footprint plane")
             w("  const uint64_t wbyte = (uint64_t)tcsg_uni(wb) * tcsg_uni(sxy);")
             w(f"  const u32 real = ((z0 - wb) * ydim + y0) * xdim * {es}u + (x0 - {XL}u) * {es}u;")
         else:
@@ -681,12 +701,13 @@ class _Emitter:
                 w(f"  __shared__ TV tcsg_stage[{BZ * 2 * NROW * TX}];")
             else:
                 w("  auto& tcsg_stage = tcsg_emu_stage;")
-            w(f"  const u32 yb = byi * {TY}u;  // the block's first footprint row")
+            if not sremain:
+                w(f"  const u32 yb = byi * {TY}u;  // the block's first footprint row")
             w(f"  const u32 sread = (threadIdx.z * {2 * NROW}u + threadIdx.y * {self.YC}u) * {TX}u + threadIdx.x * {XC}u;")
             w(f"  const u32 swrite = (threadIdx.z * {2 * NROW}u + threadIdx.y) * {TX}u + threadIdx.x * {XC}u;")
             w("  // any row and column of the tile can feed a valid output: every lane of a valid z chunk loads")
-            w(f"  const u32 stbase = tcsg_sel(gz < thz, ((z0 - wb) * ydim + yb + threadIdx.y) * xdim * {es}u "
-              f"+ (x0 - {XL}u) * {es}u, oob);")
+            w(f"  const u32 stbase = tcsg_sel({'inrow' if sremain else 'gz < thz'}, ((z0 - wb) * ydim + yb + "
+              f"threadIdx.y) * xdim * {es}u + (x0 - {XL}u) * {es}u, oob);")
         pts, contrib = self.footprint()
         # edge_pairs: two adjacent rows of one plane that both need edge columns share a load;
         # the second row's columns go to the next H lanes on each side (one row below in the

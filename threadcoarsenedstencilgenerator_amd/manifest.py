@@ -244,6 +244,18 @@ def staged_cases():
     ]
 
 
+def staged_remainder_cases():
+    """The staged kernels generated with remainder handling, on grids whose interiors are not
+    multiples of the coarsening nor of the block's tile in any axis (the last block of rows and
+    of planes starts early, the last lanes in x drop their columns past the interior), each
+    interior at least one tile in y and z."""
+    sizes = [(223, 29, 21), (301, 45, 18), (151, 19, 14), (303, 15, 13), (201, 9, 11), (141, 21, 13)]
+    out = []
+    for (d, _, block), size in zip(staged_cases(), sizes):
+        out.append((d.replace(remainder=True), size, block))
+    return out
+
+
 def smoke_desc():
     """3D Star Variable r=4, coarsening 1x4x16, DPP x exchange (the smoke run's kernel)."""
     return cfg2_desc(1, 4, 16, shape="dpp", max_threads=256)
@@ -267,7 +279,7 @@ def default_build_set():
     descs += [cfg4_desc(dt, "plain", batched=True, max_threads=1024).replace(remainder=True) for dt in (F64, F32C)]
     descs += [cfg4_desc(dt, "dppw", batched=True).replace(remainder=True) for dt in (F64, F32C)]
     descs += [c[1] for c in store_hazard_cases()]
-    descs += [d for d, _, _ in staged_cases()]
+    descs += [d for d, _, _ in staged_cases()] + [d for d, _, _ in staged_remainder_cases()]
     from . import tuned, tuner
     descs += tuned.build_set()
     descs += [d for d, _ in tuner.live_configs()]   # the tuner's live check

@@ -130,10 +130,10 @@ class StencilDesc:
             raise ValueError("stage is a block shape (bx, by, bz), or (0, 0, 0)")
         if self.staged:
             bx, by, bz = self.stage
-            if (self.variant != STREAM or self.dimension != 3 or self.xext != "dppw" or self.remainder
+            if (self.variant != STREAM or self.dimension != 3 or self.xext != "dppw"
                     or self.batched or self.prefetch):
                 raise ValueError("LDS staging is an option of the 3D wave-aligned DPP STREAM shape "
-                                 "(without remainder handling, batching or register prefetch)")
+                                 "(without batching or register prefetch)")
             if bx % 64 or not 64 <= bx <= 1024 or not 1 <= by <= 31 or not 1 <= bz <= 31:
                 raise ValueError("staged block: bx a multiple of 64 up to 1024, by and bz in [1, 31]")
             if bx * by * bz > self.max_threads:

@@ -29,16 +29,22 @@ CFG2_F64 = dict(desc=manifest.cfg2_desc(2, 4, 128, shape="dppw", max_threads=512
 # the output's own plane, the second the 4 terms of the planes after it, and TwoSum joins them.
 CFG2_F32C = dict(desc=manifest.cfg2_desc(4, 4, 64, shape="dppw", max_threads=256, dtype=F32C).replace(
     prefetch=0, comp_group=21, zfirst=False, cl_size=32, edge_pairs=True, stage=(64, 4, 1)), block=(64, 4, 1))
-# the kernels that load their own rows (no LDS staging), fastest before staging: remainder handling
-# (a 511^3 interior) runs these, as staging covers whole coarsening tiles only
+# the kernels that load their own rows (no LDS staging), fastest before staging (the staged
+# kernels' baselines in the measurements)
 CFG2_F64_UNSTAGED = dict(desc=CFG2_F64["desc"].replace(stage=(0, 0, 0)), block=(128, 2, 2))
 CFG2_F32C_UNSTAGED = dict(desc=manifest.cfg2_desc(2, 8, 64, shape="dppw", max_threads=256, dtype=F32C).replace(
     prefetch=0, comp_group=21, zfirst=False, cl_size=32, edge_pairs=True), block=(128, 2, 1))
 
 
-def remainder_cfg(dtype: int = F64):
-    """The 512^3 workload's kernel generated with remainder handling."""
-    c = CFG2_F32C_UNSTAGED if dtype == F32C else CFG2_F64_UNSTAGED
+def remainder_cfg(dtype: int = F64, staged: bool = True):
+    """The 512^3 workload's kernel generated with remainder handling: the staged production
+    kernel (its last block of rows and of planes starts early enough to end at the interior's
+    end), or with staged=False the kernel that loads its own rows (its last thread of each
+    axis starts early instead)."""
+    if staged:
+        c = CFG2_F32C if dtype == F32C else CFG2_F64
+    else:
+        c = CFG2_F32C_UNSTAGED if dtype == F32C else CFG2_F64_UNSTAGED
     return dict(desc=c["desc"].replace(remainder=True), block=c["block"])
 
 
@@ -120,5 +126,6 @@ def build_set():
     return [CFG2_F64["desc"], CFG2_F32C["desc"], BOX2D_R8_F64["desc"], BOX2D_R8_F32C["desc"],
             CFG2_F64_UNSTAGED["desc"], CFG2_F32C_UNSTAGED["desc"],
             remainder_cfg(F64)["desc"], remainder_cfg(F32C)["desc"],
+            remainder_cfg(F64, staged=False)["desc"], remainder_cfg(F32C, staged=False)["desc"],
             BOX2D_R1_TEMPORAL["desc"], BOX2D_R1_ONE_STEP["desc"], BOX2D_R1_TEMPORAL_F32C["desc"],
             STAR3D_R1_TEMPORAL["desc"], STAR3D_R1_ONE_STEP["desc"], BATCHED_F64["desc"], BATCHED_F32C["desc"]]

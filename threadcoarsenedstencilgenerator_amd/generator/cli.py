@@ -17,7 +17,8 @@ shape (no reference counterpart) are given as --name=value:
 updated whole instead of truncated), --edge-pairs (--xext=dppw: the wave-edge
 lanes fetch two adjacent rows' extra columns per load), --stage=BXxBYxBZ (3D,
 --xext=dppw: LDS halo staging for blocks of BX x BY x BZ threads, the only
-block shape such a kernel is then launched with; its source has no reference
+block shape such a kernel is then launched with; with --remainder the grid's
+interior must hold one block's tile in y and z; its source has no reference
 updateGrid, which takes any block shape).
 Invalid arguments print the usage on stderr and exit with status 1, as the
 reference does.

@@ -673,6 +673,8 @@ def live_evaluate(configs=None, k: int = 8, model: "CostModel | None" = None, lo
     own = model is None
     model = CostModel.load() if own else model
     rep = evaluate(Corpus([]), Corpus(rows), k=k, model=model, mix=model.mix, scores=model.fixed)
+    for key in ("train_rows", "train_groups", "fit_seconds"):   # the committed model is not trained here
+        rep.pop(key)
     rep["mix"] = model.mix
     if own:
         model.close()

@@ -167,6 +167,27 @@ def test_top_k_outcome():
     assert reg == 0.0
 
 
+def test_mixed_order():
+    """The k shapes tune() measures: the model's first k - mix, then the model-free ranking's
+    first not already chosen; the rest follows in the model's order, every shape once."""
+    pred, fixed = np.array([3, 1, 2, 0, 4]), np.array([1, 4, 0, 2, 3])
+    assert list(tuner.mixed_order(pred, fixed, 4, 0)) == [3, 1, 2, 0, 4]
+    assert list(tuner.mixed_order(pred, fixed, 4, 1)) == [3, 1, 2, 4, 0]
+    assert list(tuner.mixed_order(pred, fixed, 4, 2)) == [3, 1, 4, 0, 2]
+    assert list(tuner.mixed_order(pred, fixed, 4, 4)) == [1, 4, 0, 2, 3]
+    assert list(tuner.mixed_order(pred, fixed, 8, 1)) == [3, 1, 2, 0, 4]   # k past the candidates
+
+
+def test_live_configurations_are_fresh():
+    """The live check's configurations: distinct kernels in neither measured corpus's
+    configuration set, drawn from the same workload space."""
+    live = tuner.live_configs()
+    assert len(live) == tuner.LIVE_CONFIGS
+    keys = [d.key() for d, _ in live]
+    assert len(set(keys)) == len(keys)
+    assert not set(keys) & {d.key() for d, _ in tuner.train_configs() + tuner.heldout_configs()}
+
+
 def test_corpus_files_match_their_configuration_sets():
     """The committed corpora: the held-out file holds one whole sweep per
     held-out configuration (every launchable shape once), the training file

@@ -562,7 +562,7 @@ class _Emitter:
             w("  const lbool inrow = gy < thy && gz < thz;")
         w("  const lbool valid = inrow && gx < thx;")
         w(f"  const u32 x0 = gx * {XC}u + {R + LO}u;")
-        if d.remainder and (self.YC > 1 or (self.three and self.ZC > 1)):
+        if d.remainder and not sremain and (self.YC > 1 or (self.three and self.ZC > 1)):
             w("  // remainder: the last thread of an axis whose interior is not a multiple of the")
             w("  // coarsening starts early enough to end at the interior's end; the outputs it")
             w("  // shares with its neighbour get the same value from both (same inputs, same")

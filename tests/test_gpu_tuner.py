@@ -117,25 +117,6 @@ def test_top8_on_heldout_configurations(tuner, corpora):
     assert mr["mean"] < fr["mean"]
 
 
-def test_live_top8_on_fresh_configurations(tuner):
-    """The committed model on 40 configurations whose kernels neither corpus contains,
-    swept exhaustively on this GPU now (nothing about them was measured before): timing
-    only its top 8 of each configuration's launchable shapes lands within 5% of the sweep's
-    best on average, in at most 10% of the sweeps' GPU time, and loses less than 8 random
-    shapes would."""
-    if not os.path.exists(tuner.DEFAULT_MODEL) or not os.path.exists(tuner.TRAIN_DATA):
-        pytest.skip("no trained cost model or training corpus in this tree")
-    print(f"case: live top-8 on {tuner.LIVE_CONFIGS} fresh configurations", flush=True)
-    rep = tuner.live_evaluate(k=8, log=lambda m: print(m, flush=True))
-    mr, rr = rep["model_regret"], rep["random_regret"]
-    print({k: rep[k] for k in ("model_regret", "random_regret", "fixed_regret", "model_time_frac_total",
-                               "within_5pct", "wall_seconds_sweeps")}, flush=True)
-    assert rep["heldout_groups"] == tuner.LIVE_CONFIGS
-    assert mr["mean"] <= 0.05
-    assert rep["model_time_frac_total"] <= 0.10
-    assert mr["mean"] < 0.5 * rr["mean"]
-
-
 def test_tune_with_the_committed_model_on_unseen_kernels(tuner):
     """tune() with the committed model on four kernels outside both corpora
     (the reference's code shape with launch bound 1024: 2D Box Jacobi r=1

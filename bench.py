@@ -665,7 +665,7 @@ def other_workloads(args, others):
     fails is reported with its error instead."""
     import torch
     out = {}
-    for name, steps in (("box2d_r1_temporal", 20), ("star3d_r1_temporal", 20), ("box2d_r8", 50),
+    for name, steps in (("box2d_r1_temporal", 100), ("star3d_r1_temporal", 100), ("box2d_r8", 50),
                         ("batched", 100), ("star3d_r4_large", 10), ("star3d_r4_511_remainder", 10)):
         sub = argparse.Namespace(**vars(args))
         sub.steps, sub.warmup, sub.no_cpu_baseline = steps, 2, True

@@ -129,14 +129,16 @@ def f32c_companion(k64, g, g64, steps, stream):
 def bench_batched(args):
     """64 independent 3D Box Jacobi r=2 grids, each interior axis log-uniform
     in [32, 256] (rounded to the workload's 2x2x4 coarsening), 100 time steps,
-    by the tuned kernel (the reference's code shape with z chunks of 8 planes
-    and remainder handling).  Modes: one persistent kernel per step (as many
+    by the tuned kernel (z march with x-neighbours by loads, coarsening 2x4x16,
+    remainder handling, packed tiles: runs of 256 consecutive threads of a
+    grid's xy thread space).  Modes: one persistent kernel per step (as many
     workgroups per CU as fit, taking tiles from a work queue with one range of
     the tile table per XCD) over a captured hipGraph of the step loop (the
     headline of this workload), the same without the graph, and 64 separate
     launches per step, of the same kernel and of the plain 2x2x4 kernel (the
-    workload's named baseline).  The *_unrounded entries run the same draws
-    without the rounding."""
+    workload's named baseline); also the previous tuned kernel (the
+    reference's code shape, z chunks of 8, rectangular tiles) persistent +
+    graph.  The *_unrounded entries run the same draws without the rounding."""
     import torch
 
     from threadcoarsenedstencilgenerator_amd import manifest, tuned
@@ -187,6 +189,14 @@ def bench_batched(args):
                 pts * steps / timed(lambda: b0.run_separate(steps, stream=stream)) / 1e9, 2)
             b0.close()
             k0.close()
+            prev = tuned.BATCHED_PLAIN_F32C if dt == F32C else tuned.BATCHED_PLAIN_F64
+            k1 = rt.Kernel(prev["desc"])
+            b1 = rt.Batch(k1, [k1.geometry(sz) for sz in sizes], [g.ptr(0) for g in grids],
+                          [g.ptr(1) for g in grids], tuple(prev["block"]))
+            res["persistent_graph_previous_plain"] = round(
+                pts * steps / timed(lambda: b1.run(steps, graph=True, stream=stream)) / 1e9, 2)
+            b1.close()
+            k1.close()
         out[name] = {"glups": res, "points": pts, "tiles": tiles, "workgroups": info, "kernel": d.key(),
                      "block": list(block),
                      "speedup_persistent_graph_vs_separate": round(res["persistent_graph"] / res["separate_launches"], 3)}
@@ -208,8 +218,9 @@ def bench_batched(args):
             "unit": "GLup/s", "n_gpus": 1, "steps": steps, "higher_is_better": True, "dtype": "f64",
             "data": "synthetic", "config": {"workload": "64 grids, interior axes log-uniform in [32,256] rounded "
                                             "to coarsening 2x2x4, U[0,1) values", "block": list(block),
-                                            "code_shape": "plain, z chunks of 8 with remainder handling "
-                                                          "(persistent batched kernel)"},
+                                            "code_shape": "z march (x-neighbours by loads), coarsening 2x4x16, "
+                                                          "remainder handling, packed tiles (persistent "
+                                                          "batched kernel)"},
             "detail": out}
     return line
 
@@ -690,6 +701,7 @@ def other_workloads(args, others):
             summ["f32c"] = d["f32c"]["glups"]["persistent_graph"]
             summ["separate_launches"] = d["f64"]["glups"]["separate_launches"]
             summ["separate_launches_2x2x4"] = d["f64"]["glups"]["separate_launches_2x2x4"]
+            summ["previous_plain_kernel"] = d["f64"]["glups"]["persistent_graph_previous_plain"]
         if name == "box2d_r8":
             summ["f32c"] = ln["f32c"]["glups"]
         if name == "star3d_r4_large":

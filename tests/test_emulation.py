@@ -92,6 +92,51 @@ def test_emulated_batched_kernel_matches_oracle(dtype, nwg):
         assert np.array_equal(g.view(ui), r.view(ui))
 
 
+@pytest.mark.parametrize("dtype", ["f64", "f32c"])
+def test_emulated_packed_batched_kernel_matches_oracle(dtype):
+    """Packed tiles (runs of block x * block y consecutive threads of a grid's
+    xy thread space, continuing on the next thread row) over uneven grids whose
+    thread rows the block's x extent does not divide, with remainder handling
+    on every axis: every grid equals the untruncated oracle after 3 steps
+    (STREAM, x-neighbours by loads; the CPU emulation covers STREAM kernels)."""
+    dt_ = F32C if dtype == "f32c" else spec.F64
+    d = manifest.cfg4_desc(shape="stream", batched=True, dtype=dt_).replace(remainder=True, pack=True, zc=8)
+    r = d.range
+    sizes = [(2 * r + 13, 2 * r + 7, 2 * r + 9), (2 * r + 30, 2 * r + 11, 2 * r + 17), (2 * r + 5, 2 * r + 21, 2 * r + 8),
+             (2 * r + 46, 2 * r + 3, 2 * r + 12)]
+    dt = np.float32 if dt_ == F32C else np.float64
+    run = oracle.run_f32c if dt_ == F32C else oracle.run_f64
+    geoms, a0, a1, refs = [], [], [], []
+    for i, size in enumerate(sizes):
+        geo = spec.geometry(d, size)
+        og = oracle.Geometry(3, r, d.alignment, d.cl_size, size, (1, 1, 1))
+        a = np.zeros(og.n_points)
+        oracle.init_seeded(og, 0, 30 + i, a)
+        a = a.astype(dt)
+        geoms.append(geo)
+        a0.append(a.copy())
+        a1.append(a.copy())
+        refs.append(run(d, og, None, a.copy(), a.copy(), 3))
+    got, bad = emulate.run_batched(d, geoms, a0, a1, None, 3, (32, 4, 1), 7)
+    assert bad == 0
+    ui = np.uint32 if dt == np.float32 else np.uint64
+    for g, ref in zip(got, refs):
+        assert np.array_equal(g.view(ui), ref.view(ui))
+
+
+def test_pack_refuses_lane_exchange():
+    """Packed tiles put lanes of one wave on different thread rows, so code
+    shapes that pass x-neighbours between lanes (DPP), LDS staging, 2D and
+    unbatched kernels refuse the option."""
+    base = manifest.cfg4_desc(shape="stream", batched=True).replace(pack=True)
+    assert base.pack
+    manifest.cfg4_desc(shape="plain", batched=True).replace(pack=True)
+    for bad in (manifest.cfg4_desc(shape="dpp", batched=True), manifest.cfg4_desc(shape="dppw", batched=True),
+                manifest.cfg4_desc(shape="stream", batched=False)):
+        with pytest.raises(ValueError):
+            bad.replace(pack=True)
+
+
 def _remainder_cases():
     """Grids whose interiors are not multiples of the coarsening on at least
     one axis (every axis for some), generated with remainder handling."""

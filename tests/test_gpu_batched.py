@@ -17,17 +17,23 @@ pytestmark = pytest.mark.gpu
 @pytest.mark.parametrize("shape,block,workgroups", [("dppw", (64, 2, 1), 7), ("plain", (32, 4, 2), 7),
                                                     ("plain", (32, 4, 2), 0), ("plain", (32, 4, 2), -1),
                                                     ("plain+remainder", (32, 4, 2), 0),
-                                                    ("dppw+remainder", (64, 2, 1), 7)])
+                                                    ("dppw+remainder", (64, 2, 1), 7),
+                                                    ("stream+remainder+pack", (32, 4, 1), 7),
+                                                    ("stream+remainder+pack", (32, 4, 1), 0),
+                                                    ("stream+remainder+pack", (16, 8, 2), -1),
+                                                    ("plain+remainder+pack", (32, 4, 2), 0)])
 def test_batched_matches_oracle(dtype, mode, shape, block, workgroups):
     """workgroups: 7 (fewer than 8: every workgroup strides the whole table),
-    0 (as many per CU as fit: the per-XCD ranges), -1 (one per tile)."""
+    0 (as many per CU as fit: the per-XCD ranges), -1 (one per tile).  pack:
+    tiles are runs of block x * block y consecutive threads of a grid's xy
+    thread space (x fastest, continuing on the next thread row)."""
     import torch
 
     from threadcoarsenedstencilgenerator_amd import runtime as rt
-    shape, _, rem = shape.partition("+")
+    shape, *flags = shape.split("+")
+    rem = "remainder" in flags
     d = manifest.cfg4_desc(shape=shape, batched=True, dtype=dtype, max_threads=1024 if shape == "plain" else 256)
-    if rem:
-        d = d.replace(remainder=True)
+    d = d.replace(remainder=rem, pack="pack" in flags)
     k = rt.Kernel(d)
     # remainder kernels take any size: interiors not rounded to the coarsening
     sizes = rt.batch_sizes(6, 3, 8, 40, (1, 1, 1) if rem else d.coarsening, d.range)
@@ -42,7 +48,10 @@ def test_batched_matches_oracle(dtype, mode, shape, block, workgroups):
         grids.append(gr)
     torch.cuda.synchronize()
     init = [gr.host(0) for gr in grids]
-    if workgroups < 0:
+    if workgroups < 0 and d.pack:
+        per = block[0] * block[1]
+        workgroups = sum(-(-g.threads.x * g.threads.y // per) * -(-g.threads.z // block[2]) for g in geoms)
+    elif workgroups < 0:
         workgroups = sum(int(np.prod(rt.grid_blocks(g, block))) for g in geoms)
     b = rt.Batch(k, geoms, [gr.ptr(0) for gr in grids], [gr.ptr(1) for gr in grids], block, workgroups=workgroups)
     steps = 5
@@ -119,7 +128,7 @@ def test_graph_replays_with_work_between(dtype):
 @pytest.mark.parametrize("dtype", [F64, F32C])
 def test_benchmark_batch_matches_oracle(dtype):
     """The batch bench.py times: 64 grids with interior axes log-uniform in [32, 256]
-    (rounded to the 2x2x4 coarsening), the benchmarked kernel (z chunks of 8 planes with
+    (rounded to the 2x2x4 coarsening), the benchmarked kernel (z march over packed tiles with
     remainder handling) and block, the occupancy-sized persistent grid (one range of the
     tile table per XCD, workgroups draining other XCDs' ranges, the last workgroup clearing
     the queue), 3 steps through the captured graph, replayed twice from the restored state:

@@ -247,6 +247,9 @@ struct tcsg_kernel {
     uint32_t tsteps = 1;     // time steps one launch advances (temporal kernels: params[18] bits 8-15)
     tcsg_dim3 stage{0, 0, 0};  // STREAM with LDS staging: the block shape it stages for (params[18]
                                // bit 3; bx/64, by, bz in bits 16-20, 21-25, 26-30), else 0
+    bool pack = false;  // batched kernel over packed tiles (params[23] bit 1): a tile is a run of
+                        // block.x * block.y consecutive threads of a grid's xy thread space (x
+                        // fastest) times block.z thread planes
 };
 
 extern "C" {
@@ -330,6 +333,9 @@ int tcsg_kernel_load(tcsg_kernel **out, const char *path, const tcsg_desc *expec
     k->xext_mode = d.variant == TCSG_STREAM ? (int)q[16] : 0;
     if (k->xext_mode > 2) return fail(TCSG_ERR_MISMATCH);
     k->remainder = (q[18] & 4u) != 0;
+    k->pack = (q[23] & 2u) != 0;
+    if (k->pack && (d.dimension != 3 || k->xext_mode != 0 || (q[18] & 8u) || !(q[23] & 1u)))
+        return fail(TCSG_ERR_MISMATCH);
     k->tsteps = d.variant == TCSG_TEMPORAL ? (q[18] >> 8) & 0xFFu : 1u;
     if (k->tsteps < 1 || (d.variant == TCSG_TEMPORAL && q[15] % 64)) return fail(TCSG_ERR_MISMATCH);
     if (q[18] & 8u) {
@@ -759,9 +765,18 @@ int tcsg_batch_create(tcsg_batch **out, tcsg_kernel *k, uint32_t n, const tcsg_g
                        p ? bufs0[i] : bufs1[i], k->mask, a);
             args[p].push_back(a);
         }
-        for (uint32_t z = 0; z < nb.z; ++z)
-            for (uint32_t y = 0; y < nb.y; ++y)
-                for (uint32_t x = 0; x < nb.x; ++x) tiles.push_back({i, x, y, z});
+        if (k->pack) {
+            // runs of block.x * block.y threads over the grid's xy thread space, per block.z planes
+            const uint64_t nxy = (uint64_t)geoms[i].threads.x * geoms[i].threads.y;
+            const uint64_t per = (uint64_t)block.x * block.y;
+            const uint32_t nr = (uint32_t)((nxy + per - 1) / per);
+            for (uint32_t z = 0; z < nb.z; ++z)
+                for (uint32_t x = 0; x < nr; ++x) tiles.push_back({i, x, 0, z});
+        } else {
+            for (uint32_t z = 0; z < nb.z; ++z)
+                for (uint32_t y = 0; y < nb.y; ++y)
+                    for (uint32_t x = 0; x < nb.x; ++x) tiles.push_back({i, x, y, z});
+        }
     }
     b->ntiles = (uint32_t)tiles.size();
     uint32_t nwg = workgroups;

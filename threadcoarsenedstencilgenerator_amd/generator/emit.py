@@ -164,7 +164,7 @@ class _Emitter:
           f"{'f32c' if self.f32 else 'f64'} {'stream' if d.variant == STREAM else 'plain'} "
           f"xext={d.xext} xcd_remap={int(d.xcd_remap)} nt={int(d.nt_store)}"
           + (" nt_load=1" if d.nt_load else "") + (" order=offset" if d.order == "offset" else "")
-          + (" remainder=1" if d.remainder else "")
+          + (" remainder=1" if d.remainder else "") + (" pack=1" if d.pack else "")
           + (" (CPU emulation build)" if cpu else ""))
         if cpu:
             w('#include "cpu_emu.h"')
@@ -401,6 +401,16 @@ class _Emitter:
     # ------------------------------------------------------------ body parts
     def block_coords(self):
         w = self.w
+        if self.batched_body and self.d.pack:
+            w("  // packed tile: run tl.bx of blockDim.x * blockDim.y consecutive threads of the grid's")
+            w("  // xy thread space (x fastest, a run continues on the next thread row), thread planes")
+            w("  // from tl.bz * blockDim.z on")
+            w("  const u32 bzi = tl.bz;")
+            w("  const u32 lin = tl.bx * (blockDim.x * blockDim.y) + threadIdx.y * blockDim.x + threadIdx.x;")
+            w("  // (a grid has packed tiles only if thx * thy > 0)")
+            w("  const u32 gx = lin % thx, gy = lin / thx;")
+            w("  const u32 gz = bzi * blockDim.z + threadIdx.z;")
+            return
         if self.batched_body:
             w("  const u32 bxi = tl.bx, byi = tl.by, bzi = tl.bz;")
         elif self.d.xcd_remap:
@@ -467,8 +477,17 @@ class _Emitter:
             w(f"  const size_t z = (size_t)gz * {self.ZC} + {R};" if self.three else "  const size_t z = 0;")
         w("  const size_t sx = xdim;")
         w("  const size_t sxy = (size_t)xdim * ydim;")
-        w(f"  const {T}* __restrict__ src = a.in + z * sxy + y * sx + x;")
-        w(f"  {T}* __restrict__ dst = a.out + z * sxy + y * sx + x;")
+        if self.batched_body:
+            # the grid's pointers come from the grid table in memory, where the compiler cannot
+            # tell their address space: cast them to global so its loads and stores are global,
+            # not flat, instructions
+            T = f"tcsg_g{T}"
+            w(f"  const {T}* __restrict__ src = (const {T}*)(a.in + z * sxy + y * sx + x);")
+            w(f"  {T}* __restrict__ dst = ({T}*)(a.out + z * sxy + y * sx + x);")
+        else:
+            w(f"  const {T}* __restrict__ src = a.in + z * sxy + y * sx + x;")
+            w(f"  {T}* __restrict__ dst = a.out + z * sxy + y * sx + x;")
+        self.PT = T
         self.weights()
         if d.order == "offset":
             return self.body_plain_offset_order()
@@ -486,7 +505,7 @@ class _Emitter:
                 w(f"  const {T}* __restrict__ q_{_n(py)}_{_n(pz)} = src + (ptrdiff_t)({py}) * (ptrdiff_t)sx"
                   f" + (ptrdiff_t)({pz}) * (ptrdiff_t)sxy;")
             v = f"v_{_n(px)}_{_n(py)}_{_n(pz)}"
-            w(f"  const {T} {v} = q_{_n(py)}_{_n(pz)}[{px}];")
+            w(f"  const {self.T} {v} = q_{_n(py)}_{_n(pz)}[{px}];")
             for (o, t, m) in sorted(contrib[p], key=lambda c: c[0]):
                 w("  " + self.acc_stmt(o, t, m, v))
             for o in self.outputs():
@@ -501,7 +520,7 @@ class _Emitter:
         the reference's order; the weights are needed one after another (so a
         large mask streams through the scalar registers once), and a footprint
         point is loaded just before its first use and dies after its last."""
-        T = self.T
+        T = self.PT
         w = self.w
         loaded, rows = set(), set()
         outs = self.outputs()
@@ -516,7 +535,7 @@ class _Emitter:
                       f" + (ptrdiff_t)({pz}) * (ptrdiff_t)sxy;")
                 if v not in loaded:
                     loaded.add(v)
-                    w(f"  const {T} {v} = q_{_n(py)}_{_n(pz)}[{px}];")
+                    w(f"  const {self.T} {v} = q_{_n(py)}_{_n(pz)}[{px}];")
                 w("  " + self.acc_stmt(o, t, m, v))
         for o in outs:
             i, j, k = o
@@ -1522,6 +1541,8 @@ class _Emitter:
         d = self.d
         cpu = self.target == "cpu"
         w("")
+        if not cpu:
+            w(f"typedef __attribute__((address_space(1))) {self.T} tcsg_g{self.T};")
         w("struct tcsg_tile { uint32_t grid, bx, by, bz; };")
         w("struct tcsg_batch_args { const tcsg_kargs* grids; const tcsg_tile* tiles; uint32_t ntiles, pad;")
         w("                         uint32_t* queue; };")
@@ -1556,6 +1577,24 @@ class _Emitter:
         w("  }")
         w("  return 0xFFFFFFFFu;")
         w("}")
+        if cpu:
+            w("template <class S> static inline S tcsg_uniform(const S& v) { return v; }")
+        else:
+            w("// A value every lane of the wave holds alike, moved to scalar registers word by word.")
+            w("// The tile entry and the grid's arguments are read from memory the kernel also writes")
+            w("// (the queue, other grids), so the compiler keeps them in vector registers; the")
+            w("// buffer resources built from them must be scalar, and a vector resource would make")
+            w("// it wrap every buffer access in a loop over the distinct values of the wave's lanes.")
+            w("template <class S> __device__ __forceinline__ S tcsg_uniform(const S& v) {")
+            w("  static_assert(sizeof(S) % 4 == 0, \"whole words\");")
+            w("  uint32_t wd[sizeof(S) / 4];")
+            w("  __builtin_memcpy(wd, &v, sizeof(S));")
+            w("#pragma unroll")
+            w("  for (unsigned i = 0; i < sizeof(S) / 4; ++i) wd[i] = (uint32_t)__builtin_amdgcn_readfirstlane((int)wd[i]);")
+            w("  S u;")
+            w("  __builtin_memcpy(&u, wd, sizeof(S));")
+            w("  return u;")
+            w("}")
         w(f"{HD} void tcsg_finish(uint32_t* q, uint32_t nwg) {{")
         w("  tcsg_fence();")
         w("  // the last workgroup here: every other one is done with the queue")
@@ -1585,10 +1624,19 @@ class _Emitter:
             w("  // passed this barrier, i.e. has read the previous one")
             w("  if (leader) tcsg_slot[it & 1u] = tcsg_fetch(ba.queue, ba.ntiles, xcd, nr, s);")
             w("  __syncthreads();")
-            w("  const uint32_t t = tcsg_slot[it & 1u];")
+            if d.variant == PLAIN:
+                w("  const uint32_t t = tcsg_slot[it & 1u];")
+            else:
+                w("  // the same tile for every lane, taken as wave-uniform: the buffer resources built")
+                w("  // from the grid's arguments are then scalar values")
+                w("  const uint32_t t = (uint32_t)__builtin_amdgcn_readfirstlane((int)tcsg_slot[it & 1u]);")
             w("  if (t == 0xFFFFFFFFu) break;")
-        w("  const tcsg_tile tl = ba.tiles[t];")
-        w("  const tcsg_kargs a = ba.grids[tl.grid];")
+        if d.variant == PLAIN:
+            w("  const tcsg_tile tl = ba.tiles[t];")
+            w("  const tcsg_kargs a = ba.grids[tl.grid];")
+        else:
+            w("  const tcsg_tile tl = tcsg_uniform(ba.tiles[t]);")
+            w("  const tcsg_kargs a = tcsg_uniform(ba.grids[tl.grid]);")
         self.batched_body = True
         if d.variant == PLAIN:
             w("  do {")

@@ -100,7 +100,9 @@ def run(desc: spec.StencilDesc, geom, mask, a: np.ndarray, b: np.ndarray, steps:
 
 def run_batched(desc: spec.StencilDesc, geoms, arrays0, arrays1, mask, steps: int, block, nwg: int):
     """Emulate tcsg_update_batched over several grids (tile table built as the
-    runtime builds it: grid-major, x fastest).  Returns (final arrays, bad)."""
+    runtime builds it: grid-major, x fastest; with desc.pack, runs of
+    block x * block y threads over each grid's xy thread space per block z
+    planes).  Returns (final arrays, bad)."""
     L = C.CDLL(build(desc))
     L.tcsg_emulate_batched.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32,
                                        C.c_uint32, C.c_uint32]
@@ -109,6 +111,12 @@ def run_batched(desc: spec.StencilDesc, geoms, arrays0, arrays1, mask, steps: in
     tiles = []
     for gi, g in enumerate(geoms):
         nb = [(g.threads[a] + block[a] - 1) // block[a] for a in range(3)]
+        if desc.pack:
+            per = block[0] * block[1]
+            for z in range(nb[2]):
+                for x in range((g.threads[0] * g.threads[1] + per - 1) // per):
+                    tiles.append((gi, x, 0, z))
+            continue
         for z in range(nb[2]):
             for y in range(nb[1]):
                 for x in range(nb[0]):

@@ -278,6 +278,9 @@ def default_build_set():
     # batched kernels with remainder handling (grids not rounded to the coarsening)
     descs += [cfg4_desc(dt, "plain", batched=True, max_threads=1024).replace(remainder=True) for dt in (F64, F32C)]
     descs += [cfg4_desc(dt, "dppw", batched=True).replace(remainder=True) for dt in (F64, F32C)]
+    # ... and over packed tiles
+    descs += [cfg4_desc(dt, shape, batched=True, max_threads=1024 if shape == "plain" else 256).replace(
+        remainder=True, pack=True) for dt in (F64, F32C) for shape in ("plain", "stream")]
     descs += [c[1] for c in store_hazard_cases()]
     descs += [d for d, _, _ in staged_cases()] + [d for d, _, _ in staged_remainder_cases()]
     from . import tuned, tuner

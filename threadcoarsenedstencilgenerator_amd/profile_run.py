@@ -11,7 +11,8 @@ counter pass measures both.
          --workload all --meta FILE: every measured workload in one pass, see all_workloads)
 
 With --batched-sweep it instead times the batched workload's kernel candidates
-(coarsening and block shape, each checked bit for bit against the 2x2x4 kernel).
+(code shape, coarsening, tile packing and block shape, each checked bit for bit
+against the plain 2x2x4 kernel).
 
 With --ceiling it instead times the HBM stream probes (copy, read-only,
 write-only; plain and nontemporal; 1-16 loads of 16 bytes in flight per lane,
@@ -54,30 +55,60 @@ def ceiling(reps: int = 20):
 
 
 def batched_candidates(dtype):
-    """(name, desc, block) of the batched persistent kernels the coarsening and block sweep
-    times: the reference's code shape (PLAIN, launch bound 1024 unless named) with remainder
-    handling, so any coarsening runs the workload's 2x2x4-rounded grids untruncated."""
+    """(name, desc, block) of the batched persistent kernels the code-shape, coarsening and block
+    sweep times, all with remainder handling (any coarsening runs the workload's 2x2x4-rounded
+    grids untruncated): the reference's code shape (PLAIN), the z-march shape (STREAM) with
+    x-neighbours by loads ("load"), by lane shifts within block rows ("dpp") or within whole waves
+    ("dppw"), over rectangular tiles or packed ones ("pack": runs of block x * block y consecutive
+    threads of a grid's xy thread space).  PLAIN kernels have a launch bound of 1024 threads,
+    STREAM ones 256 unless named ("_mt")."""
     from . import manifest
-    base = manifest.cfg4_desc(shape="plain", batched=True, dtype=dtype, max_threads=1024)
-    out = [("2x2x4", base, (32, 8, 2))]
-    table = (((2, 2, 8), 1024, ((32, 8, 2), (32, 16, 1), (32, 4, 4), (32, 8, 1), (32, 8, 4), (16, 16, 2),
-                               (64, 8, 1), (64, 4, 2))),
-             ((2, 2, 12), 1024, ((32, 8, 2),)), ((2, 2, 16), 1024, ((32, 8, 2),)), ((2, 2, 6), 1024, ((32, 8, 2),)),
-             ((2, 1, 8), 1024, ((32, 16, 1),)), ((1, 2, 8), 1024, ((32, 8, 2),)), ((2, 4, 4), 1024, ((32, 8, 2),)),
-             ((4, 2, 4), 1024, ((16, 8, 2),)), ((4, 4, 4), 256, ((16, 8, 2),)), ((4, 4, 2), 1024, ((16, 8, 2),)),
-             ((4, 2, 8), 256, ((16, 8, 2),)))
-    for co, mt, blocks in table:
-        d = base.replace(xc=co[0], yc=co[1], zc=co[2], remainder=True, max_threads=mt)
+    from .spec import F64
+    plain = manifest.cfg4_desc(shape="plain", batched=True, dtype=dtype, max_threads=1024)
+    out = [("plain 2x2x4", plain, (32, 8, 2))]
+    for co, blocks, pack in (((2, 2, 8), ((32, 8, 2), (16, 16, 2), (32, 4, 4)), False),
+                             ((1, 2, 8), ((32, 8, 2),), False), ((2, 2, 12), ((32, 8, 2),), False),
+                             ((2, 2, 16), ((32, 8, 2),), False), ((2, 4, 4), ((32, 8, 2),), False),
+                             ((4, 2, 4), ((16, 8, 2),), False),
+                             ((2, 2, 8), ((32, 8, 2),), True), ((1, 2, 8), ((32, 8, 2),), True)):
+        d = plain.replace(xc=co[0], yc=co[1], zc=co[2], remainder=True, pack=pack)
         for blk in blocks:
-            out.append((f"{co[0]}x{co[1]}x{co[2]}" + ("" if mt == 1024 else f"_mt{mt}"), d, blk))
+            out.append((f"plain {co[0]}x{co[1]}x{co[2]}" + (" pack" if pack else ""), d, blk))
+    shape = {"load": "stream", "dpp": "dpp", "dppw": "dppw"}
+    nt = dtype == F64
+    table = (("dppw", (2, 2, 16), 256, ((64, 4, 1),), False, 1, nt),
+             ("dpp", (2, 2, 16), 256, ((32, 8, 1), (16, 16, 1)), False, 1, nt),
+             ("dpp", (2, 4, 16), 256, ((32, 8, 1),), False, 1, nt),
+             ("load", (2, 2, 16), 256, ((32, 8, 1),), False, 1, nt),
+             ("load", (2, 4, 16), 256, ((32, 8, 1), (16, 16, 1)), False, 1, nt),
+             ("load", (2, 4, 16), 256, ((32, 8, 1), (32, 4, 2)), True, 1, nt),
+             ("load", (2, 4, 16), 256, ((32, 8, 1),), True, 1, not nt),
+             ("load", (2, 4, 16), 256, ((32, 8, 1),), True, 0, nt),
+             ("load", (2, 4, 16), 256, ((32, 8, 1),), True, 2, nt),
+             ("load", (2, 4, 16), 128, ((32, 4, 1),), True, 1, nt),
+             ("load", (2, 4, 16), 512, ((32, 16, 1),), True, 1, nt),
+             ("load", (2, 2, 16), 256, ((32, 8, 1),), True, 1, nt),
+             ("load", (2, 2, 8), 256, ((32, 8, 1),), True, 1, nt),
+             ("load", (2, 4, 8), 256, ((32, 8, 1),), True, 1, nt),
+             ("load", (2, 4, 32), 256, ((32, 8, 1),), True, 1, nt),
+             ("load", (2, 3, 16), 256, ((32, 8, 1),), True, 1, nt),
+             ("load", (2, 6, 16), 256, ((32, 8, 1),), True, 1, nt),
+             ("load", (1, 4, 16), 256, ((32, 8, 1),), True, 1, nt),
+             ("load", (4, 2, 16), 256, ((16, 8, 1),), True, 1, nt))
+    for xext, co, mt, blocks, pack, pf, ntst in table:
+        d = manifest.cfg4_desc(shape=shape[xext], batched=True, dtype=dtype, max_threads=mt).replace(
+            xc=co[0], yc=co[1], zc=co[2], remainder=True, zfirst=False, pack=pack, prefetch=pf, nt_store=ntst)
+        for blk in blocks:
+            out.append((f"{xext} {co[0]}x{co[1]}x{co[2]}" + (" pack" if pack else "") + (f" mt{mt}" if mt != 256 else "")
+                        + (f" prefetch{pf}" if pf != 1 else "") + (" nt" if ntst else ""), d, blk))
     return out
 
 
-def batched_sweep(dtype_name: str, steps: int = 20):
+def batched_sweep(dtype_name: str, steps: int = 50):
     """The batched workload (64 grids, interiors log-uniform in [32, 256] rounded to 2x2x4)
     through each candidate of batched_candidates: 2 steps from the seeded state compared bit
-    for bit with the 2x2x4 kernel's, then GLup/s over `steps` graph-replayed steps.  One JSON
-    line per candidate.  Kernels not in the tree are compiled first."""
+    for bit with the plain 2x2x4 kernel's, then GLup/s over `steps` graph-replayed steps.  One
+    JSON line per candidate.  Kernels not in the tree are compiled first."""
     import json
     import time
 
@@ -92,6 +123,7 @@ def batched_sweep(dtype_name: str, steps: int = 20):
     ui = np.uint32 if dt == spec.F32C else np.uint64
     stream = torch.cuda.current_stream()
     ref = None
+    warm = True
     for name, d, block in batched_candidates(dt):
         B.build_kernel(d, force=False)
         k = rt.Kernel(d)
@@ -105,13 +137,15 @@ def batched_sweep(dtype_name: str, steps: int = 20):
         res = [g.host(fin) for g in grids]
         same = ref is None or all(np.array_equal(a.view(ui), c.view(ui)) for a, c in zip(ref, res))
         ref = res if ref is None else ref
-        bt.run(steps, graph=True, stream=stream)
+        # the first candidate also brings the GPU to its working clocks
+        bt.run(10 * steps if warm else steps, graph=True, stream=stream)
+        warm = False
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         bt.run(steps, graph=True, stream=stream)
         torch.cuda.synchronize()
         glups = pts * steps / (time.perf_counter() - t0) / 1e9
-        print(json.dumps(dict(dtype=dtype_name, coarsening=name, block=list(block), kernel=d.key(),
+        print(json.dumps(dict(dtype=dtype_name, candidate=name, block=list(block), kernel=d.key(),
                               glups=round(glups, 2), tiles=tiles, workgroups=nwg, same_as_2x2x4=bool(same))),
               flush=True)
         bt.close()
@@ -212,8 +246,8 @@ def main():
     ap.add_argument("--workload", default="star3d_r4_512", choices=("star3d_r4_512", "box2d_r1_temporal", "all"))
     ap.add_argument("--meta", default="profile_workloads.json", help="--workload all: where to write the "
                     "dispatch table")
-    ap.add_argument("--batched-sweep", action="store_true", help="time the batched kernel's coarsening and "
-                    "block candidates (--dtype f64|f32c) instead")
+    ap.add_argument("--batched-sweep", action="store_true", help="time the batched kernel's code shape, "
+                    "coarsening and block candidates (--dtype f64|f32c) instead")
     args = ap.parse_args()
     if args.ceiling:
         return ceiling()

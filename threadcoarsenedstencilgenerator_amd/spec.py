@@ -37,7 +37,7 @@ class StencilDesc:
     cl_size mirror the reference generator argv.  dtype, variant and the
     tuning knobs (max_threads, xext, xcd_remap, nt_store, comp_group,
     prefetch, waves_per_eu, zfirst, batched, nt_load, order, remainder,
-    edge_pairs, tsteps) have no reference counterpart: they select the MI355X
+    edge_pairs, tsteps, stage, pack) have no reference counterpart: they select the MI355X
     code shape.  The TEMPORAL variant advances tsteps time steps per launch:
     each workgroup stages its tile of (64 xc) x (max_threads/64 yc) points
     plus tsteps*range halo points per side in LDS and runs the steps there
@@ -87,6 +87,12 @@ class StencilDesc:
     #                          rows of every plane once, with the y halo, into LDS (two buffers),
     #                          and each thread reads its footprint rows from there instead of
     #                          loading the y-arm rows its neighbours also load; (0, 0, 0): off
+    pack: bool = False       # batched 3D: a tile is a run of bx*by consecutive threads of the grid's
+    #                          xy thread space, taken x-fastest (a run continues on the next thread
+    #                          row), times bz thread planes, instead of a bx x by rectangle: grids
+    #                          whose thread rows the block's x extent does not divide leave no
+    #                          lanes idle.  Lanes must not exchange values (PLAIN, or STREAM with
+    #                          x-neighbours by loads)
 
     def __post_init__(self):
         object.__setattr__(self, "stage", tuple(self.stage))
@@ -140,6 +146,10 @@ class StencilDesc:
                 raise ValueError("staged block has more threads than the launch bound")
             if self.stage_lds_bytes > 163840:
                 raise ValueError(f"staged block needs {self.stage_lds_bytes} bytes of LDS (gfx950: at most 160 KiB)")
+        if self.pack and (not self.batched or self.dimension != 3 or self.staged
+                          or not (self.variant == PLAIN or (self.variant == STREAM and self.xext == "load"))):
+            raise ValueError("pack is an option of batched 3D kernels whose lanes exchange no values "
+                             "(PLAIN, or STREAM with x-neighbours by loads)")
         if self.variant == TEMPORAL:
             if not 1 <= self.tsteps <= 16:
                 raise ValueError("tsteps must be in [1, 16]")
@@ -236,7 +246,7 @@ class StencilDesc:
                 int(self.nt_store) | int(self.nt_load) << 1 | int(self.remainder) << 2 | int(self.staged) << 3
                 | (self.tsteps if self.variant == TEMPORAL else 0) << 8
                 | (self.stage[0] // 64) << 16 | self.stage[1] << 21 | self.stage[2] << 26,
-                self.comp_group, self.prefetch, self.waves_per_eu, int(self.zfirst), int(self.batched)]
+                self.comp_group, self.prefetch, self.waves_per_eu, int(self.zfirst), int(self.batched) | int(self.pack) << 1]
 
     def replace(self, **kw) -> "StencilDesc":
         return dataclasses.replace(self, **kw)
@@ -245,7 +255,7 @@ class StencilDesc:
 # knobs added after kernel names were first recorded (measured corpora refer to
 # kernels by name): left out of the name's hash while at their default
 _LATER_FIELDS = {"nt_load": False, "order": "point", "remainder": False, "edge_pairs": False, "tsteps": 1,
-                 "stage": (0, 0, 0)}
+                 "stage": (0, 0, 0), "pack": False}
 
 
 def mask_index(desc: StencilDesc, dx: int, dy: int, dz: int = 0):

@@ -98,17 +98,25 @@ STAR3D_R1_ONE_STEP = dict(desc=StencilDesc(dimension=3, neighbourhood=STAR, orie
 
 
 # 64 uneven 3D Box Jacobi r=2 grids (interior axes log-uniform in [32, 256], rounded to the
-# workload's 2x2x4 coarsening), one persistent launch per step.  The reference's code shape
-# (PLAIN, each thread loading its whole footprint) with a z chunk of 8 planes per thread, which
-# loads 12 planes for 8 outputs instead of 8 for 4; remainder handling covers the interiors of
-# 4 (mod 8) planes (the last thread of such a column starts 4 planes early; the outputs it shares
-# with its neighbour get the same bits).  Sweep over coarsenings 2x2x4-2x2x16, 4x2x4-4x4x4,
-# 2x1x8, 1x2x8, 2x2x6/12 and 2-9 block shapes each: FP64 2x2x8 at 32x8x2 104.4 GLup/s against
-# 94.3 for 2x2x4; FP32C 1x2x8 at 32x8x2 148.8 against 129.2.
+# workload's 2x2x4 coarsening), one persistent launch per step.  The z-march code shape (STREAM)
+# with x-neighbours by loads, coarsening 2x4x16 and remainder handling, over packed tiles: a tile is
+# a run of 256 consecutive threads of a grid's xy thread space, so the 32-256-point grids leave no
+# lanes idle at their x and y ends (rectangular 32x8 tiles cover 0.65 of their lanes with interior
+# points).  FP64 148.0 GLup/s against 124.2 for the same kernel over rectangular tiles and 106.1 for
+# the earlier tuned kernel (the reference's code shape, 2x2x8 at 32x8x2); FP32C 196.2 against 166.2
+# and 157.9.  Sweep over coarsenings 2x2x8-2x6x16, 4x2x16, 1x4x16, 2x4x32, x-neighbours by loads or
+# by DPP, 128-512 threads, packed and rectangular tiles (profiles: batched_bench.md).
 BATCH_ROUNDING = (2, 2, 4)
-BATCHED_F64 = dict(desc=manifest.cfg4_desc(shape="plain", batched=True, max_threads=1024).replace(
+BATCHED_F64 = dict(desc=manifest.cfg4_desc(shape="stream", batched=True, max_threads=256).replace(
+    xc=2, yc=4, zc=16, remainder=True, zfirst=False, pack=True), block=(32, 8, 1))
+BATCHED_F32C = dict(desc=manifest.cfg4_desc(shape="stream", batched=True, dtype=F32C, max_threads=256).replace(
+    xc=2, yc=4, zc=16, remainder=True, zfirst=False, nt_store=False, pack=True), block=(32, 8, 1))
+# the tuned kernels before packing: the reference's code shape (PLAIN) with a z chunk of 8 planes
+# per thread (12 planes loaded for 8 outputs instead of 8 for 4) over rectangular tiles; FP64 2x2x8
+# at 32x8x2, FP32C 1x2x8 at 32x8x2
+BATCHED_PLAIN_F64 = dict(desc=manifest.cfg4_desc(shape="plain", batched=True, max_threads=1024).replace(
     zc=8, remainder=True), block=(32, 8, 2))
-BATCHED_F32C = dict(desc=manifest.cfg4_desc(shape="plain", batched=True, dtype=F32C, max_threads=1024).replace(
+BATCHED_PLAIN_F32C = dict(desc=manifest.cfg4_desc(shape="plain", batched=True, dtype=F32C, max_threads=1024).replace(
     xc=1, zc=8, remainder=True), block=(32, 8, 2))
 
 
@@ -128,4 +136,5 @@ def build_set():
             remainder_cfg(F64)["desc"], remainder_cfg(F32C)["desc"],
             remainder_cfg(F64, staged=False)["desc"], remainder_cfg(F32C, staged=False)["desc"],
             BOX2D_R1_TEMPORAL["desc"], BOX2D_R1_ONE_STEP["desc"], BOX2D_R1_TEMPORAL_F32C["desc"],
-            STAR3D_R1_TEMPORAL["desc"], STAR3D_R1_ONE_STEP["desc"], BATCHED_F64["desc"], BATCHED_F32C["desc"]]
+            STAR3D_R1_TEMPORAL["desc"], STAR3D_R1_ONE_STEP["desc"], BATCHED_F64["desc"], BATCHED_F32C["desc"],
+            BATCHED_PLAIN_F64["desc"], BATCHED_PLAIN_F32C["desc"]]

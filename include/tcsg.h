@@ -189,7 +189,10 @@ int tcsg_time_steps(tcsg_kernel *k, const tcsg_geometry *g, tcsg_dim3 block, tcs
  * One persistent kernel per time step advances every grid of the batch: its
  * workgroups take tiles, (grid, block) pairs each the work of one block of
  * tcsg_update, from a work queue over the tile table, one contiguous range per
- * XCD (a workgroup drains its XCD's range, then the others).  geoms[i] is
+ * XCD (a workgroup drains its XCD's range, then the others).  Kernels generated
+ * with packed tiles take instead runs of block.x * block.y consecutive threads
+ * of a grid's xy thread space (x fastest, continuing on the next thread row)
+ * times block.z thread planes, so uneven grids leave no lanes idle.  geoms[i] is
  * grid i's geometry, bufs0[i]/bufs1[i] its ping-pong buffers; every tile uses
  * `block`; workgroups = 0 launches as many workgroups per compute unit as the
  * kernel's occupancy allows at that block size (at most one per tile).  The

@@ -769,6 +769,8 @@ int tcsg_batch_create(tcsg_batch **out, tcsg_kernel *k, uint32_t n, const tcsg_g
             // runs of block.x * block.y threads over the grid's xy thread space, per block.z planes
             const uint64_t nxy = (uint64_t)geoms[i].threads.x * geoms[i].threads.y;
             const uint64_t per = (uint64_t)block.x * block.y;
+            // the kernel numbers a grid's xy threads in 32 bits
+            if (nxy + per > 0xFFFFFFFFull) { delete b; return TCSG_ERR_TOO_LARGE; }
             const uint32_t nr = (uint32_t)((nxy + per - 1) / per);
             for (uint32_t z = 0; z < nb.z; ++z)
                 for (uint32_t x = 0; x < nr; ++x) tiles.push_back({i, x, 0, z});

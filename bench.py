@@ -146,6 +146,8 @@ def bench_batched(args):
     from threadcoarsenedstencilgenerator_amd.spec import F32C, F64
     out = {}
     steps = 100
+    err_steps = 20
+    ref64 = None
 
     def timed(run):
         run()
@@ -174,6 +176,16 @@ def bench_batched(args):
         b = rt.Batch(k, geoms, [g.ptr(0) for g in grids], [g.ptr(1) for g in grids], block)
         pts, tiles, nwg = b.info()
         info["persistent"] = nwg
+        err = None
+        if rounded:
+            # FP32C against FP64: both from the same seeded values, after err_steps steps
+            fin = b.run(err_steps, graph=False, stream=stream)
+            torch.cuda.synchronize()
+            if dt == F64:
+                ref64 = [gr.bufs[fin].clone() for gr in grids]
+            else:
+                err = max(float((gr.bufs[fin].double() - r).abs().max() / r.abs().max()) for gr, r in zip(grids, ref64))
+                del ref64
         res["persistent_graph"] = round(pts * steps / timed(lambda: b.run(steps, graph=True, stream=stream)) / 1e9, 2)
         res["persistent"] = round(pts * steps / timed(lambda: b.run(steps, graph=False, stream=stream)) / 1e9, 2)
         res["separate_launches"] = round(pts * steps / timed(lambda: b.run_separate(steps, stream=stream)) / 1e9, 2)
@@ -202,6 +214,9 @@ def bench_batched(args):
                      "speedup_persistent_graph_vs_separate": round(res["persistent_graph"] / res["separate_launches"], 3)}
         if rounded:
             out[name]["speedup_vs_separate_2x2x4"] = round(res["persistent_graph"] / res["separate_launches_2x2x4"], 3)
+        if err is not None:
+            out[name]["normwise_max_rel_error_vs_f64"] = {"steps": err_steps, "value": err,
+                                                          "comp_group": d.comp_group}
         del grids
         torch.cuda.empty_cache()
     # HBM roofline of the FP64 persistent + graph mode: one read and one write of every interior
@@ -702,6 +717,7 @@ def other_workloads(args, others):
             summ["separate_launches"] = d["f64"]["glups"]["separate_launches"]
             summ["separate_launches_2x2x4"] = d["f64"]["glups"]["separate_launches_2x2x4"]
             summ["previous_plain_kernel"] = d["f64"]["glups"]["persistent_graph_previous_plain"]
+            summ["f32c_normwise_max_rel_error"] = d["f32c"]["normwise_max_rel_error_vs_f64"]["value"]
         if name == "box2d_r8":
             summ["f32c"] = ln["f32c"]["glups"]
         if name == "star3d_r4_large":

@@ -102,15 +102,18 @@ STAR3D_R1_ONE_STEP = dict(desc=StencilDesc(dimension=3, neighbourhood=STAR, orie
 # with x-neighbours by loads, coarsening 2x4x16 and remainder handling, over packed tiles: a tile is
 # a run of 256 consecutive threads of a grid's xy thread space, so the 32-256-point grids leave no
 # lanes idle at their x and y ends (rectangular 32x8 tiles cover 0.65 of their lanes with interior
-# points).  FP64 148.0 GLup/s against 124.2 for the same kernel over rectangular tiles and 106.1 for
-# the earlier tuned kernel (the reference's code shape, 2x2x8 at 32x8x2); FP32C 196.2 against 166.2
-# and 157.9.  Sweep over coarsenings 2x2x8-2x6x16, 4x2x16, 1x4x16, 2x4x32, x-neighbours by loads or
+# points).  FP64 152.8 GLup/s against 128.0 for the same kernel over rectangular tiles and 106.9 for
+# the earlier tuned kernel (the reference's code shape, 2x2x8 at 32x8x2); FP32C (groups of 5) 202.8
+# against 170.2 and 154.2.  Sweep over coarsenings 2x2x8-2x6x16, 4x2x16, 1x4x16, 2x4x32, x-neighbours by loads or
 # by DPP, 128-512 threads, packed and rectangular tiles (profiles: batched_bench.md).
 BATCH_ROUNDING = (2, 2, 4)
 BATCHED_F64 = dict(desc=manifest.cfg4_desc(shape="stream", batched=True, max_threads=256).replace(
     xc=2, yc=4, zc=16, remainder=True, zfirst=False, pack=True), block=(32, 8, 1))
+# FP32C: TwoSum groups of 25 terms, one z plane of the 5x5x5 box each (218.2 GLup/s in the sweep
+# against 202.8 for groups of 5; normwise error against the FP64 kernel after 20 steps 1.20e-6
+# against 1.07e-6, both set by the FP32 rounding of the stored U[0,1) state at every step)
 BATCHED_F32C = dict(desc=manifest.cfg4_desc(shape="stream", batched=True, dtype=F32C, max_threads=256).replace(
-    xc=2, yc=4, zc=16, remainder=True, zfirst=False, nt_store=False, pack=True), block=(32, 8, 1))
+    xc=2, yc=4, zc=16, remainder=True, zfirst=False, nt_store=False, pack=True, comp_group=25), block=(32, 8, 1))
 # the tuned kernels before packing: the reference's code shape (PLAIN) with a z chunk of 8 planes
 # per thread (12 planes loaded for 8 outputs instead of 8 for 4) over rectangular tiles; FP64 2x2x8
 # at 32x8x2, FP32C 1x2x8 at 32x8x2

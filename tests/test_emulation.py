@@ -124,6 +124,36 @@ def test_emulated_packed_batched_kernel_matches_oracle(dtype):
         assert np.array_equal(g.view(ui), ref.view(ui))
 
 
+@pytest.mark.parametrize("dtype", ["f64", "f32c"])
+def test_emulated_packed_star_variable_batch_matches_oracle(dtype):
+    """Packed tiles with a Variable mask, range 3, Write alignment and coarsening 2x3x5 with
+    remainder handling, over grids whose thread rows no block row divides: every grid equals
+    the oracle after 2 steps."""
+    dt_ = F32C if dtype == "f32c" else spec.F64
+    d = manifest.packed_star_desc(dt_)
+    r = d.range
+    sizes = [(2 * r + 11, 2 * r + 4, 2 * r + 6), (2 * r + 25, 2 * r + 7, 2 * r + 11), (2 * r + 6, 2 * r + 13, 2 * r + 5)]
+    dt = np.float32 if dt_ == F32C else np.float64
+    run = oracle.run_f32c if dt_ == F32C else oracle.run_f64
+    mask = oracle.mask_seeded(8, d.n)
+    geoms, a0, a1, refs = [], [], [], []
+    for i, size in enumerate(sizes):
+        geo = spec.geometry(d, size)
+        og = oracle.Geometry(3, r, d.alignment, d.cl_size, size, (1, 1, 1))
+        a = np.zeros(og.n_points)
+        oracle.init_seeded(og, 1, 40 + i, a, halo=0.25)
+        a = a.astype(dt)
+        geoms.append(geo)
+        a0.append(a.copy())
+        a1.append(a.copy())
+        refs.append(run(d, og, mask, a.copy(), a.copy(), 2))
+    got, bad = emulate.run_batched(d, geoms, a0, a1, mask.astype(dt), 2, (32, 2, 1), 5)
+    assert bad == 0
+    ui = np.uint32 if dt == np.float32 else np.uint64
+    for g, ref in zip(got, refs):
+        assert np.array_equal(g.view(ui), ref.view(ui))
+
+
 def test_pack_refuses_lane_exchange():
     """Packed tiles put lanes of one wave on different thread rows, so code
     shapes that pass x-neighbours between lanes (DPP), LDS staging, 2D and

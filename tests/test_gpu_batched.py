@@ -77,6 +77,44 @@ def test_batched_matches_oracle(dtype, mode, shape, block, workgroups):
 
 
 @pytest.mark.parametrize("dtype", [F64, F32C])
+@pytest.mark.parametrize("block,workgroups", [((32, 8, 1), 0), ((64, 2, 2), 5)])
+def test_packed_star_variable_batch_matches_oracle(dtype, block, workgroups):
+    """Packed tiles with a Variable mask (set once for the kernel, shared by every grid), range
+    3, Write alignment and coarsening 2x3x5 with remainder handling, on uneven grids not rounded
+    to the coarsening, 4 steps through the graph: every grid equals the oracle."""
+    import torch
+
+    from threadcoarsenedstencilgenerator_amd import runtime as rt
+    d = manifest.packed_star_desc(dtype)
+    k = rt.Kernel(d)
+    mask = rt.mask_seeded(8, d.n)
+    k.set_mask(mask)
+    sizes = rt.batch_sizes(7, 5, 12, 60, (1, 1, 1), d.range)
+    print(f"packed star batch {d.key()} block {block} sizes {sizes}", flush=True)
+    geoms, grids = [], []
+    for i, size in enumerate(sizes):
+        g = k.geometry(size)
+        gr = rt.Grids(g, k.dtype)
+        for j in range(2):
+            k.init(g, gr.ptr(j), rt.INIT_NORMAL, 50 + i)
+        geoms.append(g)
+        grids.append(gr)
+    torch.cuda.synchronize()
+    init = [gr.host(0) for gr in grids]
+    b = rt.Batch(k, geoms, [gr.ptr(0) for gr in grids], [gr.ptr(1) for gr in grids], block, workgroups=workgroups)
+    steps = 4
+    fin = b.run(steps, graph=True)
+    torch.cuda.synchronize()
+    run = oracle.run_f32c if dtype == F32C else oracle.run_f64
+    ui = np.uint32 if dtype == F32C else np.uint64
+    for size, gr, a in zip(sizes, grids, init):
+        og = oracle.Geometry(3, d.range, d.alignment, d.cl_size, size, (1, 1, 1))
+        ref = run(d, og, mask, a.copy(), a.copy(), steps)
+        assert np.array_equal(gr.host(fin).view(ui), ref.view(ui)), f"grid {size} differs"
+    b.close()
+
+
+@pytest.mark.parametrize("dtype", [F64, F32C])
 def test_graph_replays_with_work_between(dtype):
     """The captured step loop replayed four times from the same initial state,
     with other GPU work (the plain kernel on another grid, a torch reduction)

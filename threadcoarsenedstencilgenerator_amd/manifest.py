@@ -197,6 +197,14 @@ def cfg4_desc(dtype=F64, shape="dpp", batched=False, max_threads=256):
     return d.replace(batched=batched) if batched else d
 
 
+def packed_star_desc(dtype=F64):
+    """A batched 3D Star Variable r=3 kernel over packed tiles (x-neighbours by loads, remainder
+    handling, Write alignment): the weights and a wider footprint through the packed tile map."""
+    return StencilDesc(dimension=3, neighbourhood=STAR, orientation=ROW, coeff=VARIABLE, range=3, xc=2, yc=3, zc=5,
+                       alignment=ALIGN_WRITE, cl_size=16, dtype=dtype, comp_group=7, max_threads=256,
+                       remainder=True, batched=True, pack=True, **SHAPES["stream"])
+
+
 def store_hazard_cases():
     """(name, desc, grid size incl. halo, block, steps): the STREAM kernels whose
     16-byte stores were once followed at once by a write of their data registers,
@@ -281,6 +289,7 @@ def default_build_set():
     # ... and over packed tiles
     descs += [cfg4_desc(dt, shape, batched=True, max_threads=1024 if shape == "plain" else 256).replace(
         remainder=True, pack=True) for dt in (F64, F32C) for shape in ("plain", "stream")]
+    descs += [packed_star_desc(dt) for dt in (F64, F32C)]
     descs += [c[1] for c in store_hazard_cases()]
     descs += [d for d, _, _ in staged_cases()] + [d for d, _, _ in staged_remainder_cases()]
     from . import tuned, tuner

@@ -55,8 +55,12 @@ def test_generator_argv_like_reference():
     r = _gen("generate3DStencil.py", "Star", "Row", "Variable", "4", "2", "4", "8", "Write", "16", "--xext=dppw",
              "--edge-pairs")
     assert r.returncode == 0 and "eoffP" in r.stdout and "tcsg_rowshl2(eP_" in r.stdout   # H = ceil(4 / 2)
-    for extra in (("--order=offset",), ("--edge-pairs",)):
-        r = _gen("generate2DStencil.py", "Box", "Row", "Jacobi", "1", "2", "1", *extra)
+    # the offset order in a 2D STREAM kernel (register prefetch off by default there)
+    r = _gen("generate2DStencil.py", "Box", "Row", "Variable", "8", "4", "4", "--xext=dppw", "--order=offset")
+    assert r.returncode == 0 and "order=offset" in r.stdout and "tcsg_shr1_or" in r.stdout
+    for script, extra in (("generate3DStencil.py", ("1", "--order=offset")), ("generate2DStencil.py", ("--edge-pairs",)),
+                          ("generate2DStencil.py", ("--order=offset", "--prefetch=1"))):
+        r = _gen(script, "Box", "Row", "Jacobi", "1", "2", "1", *extra)
         assert r.returncode == 1 and "usage" in r.stderr
 
 

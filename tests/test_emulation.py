@@ -92,6 +92,33 @@ def test_emulated_batched_kernel_matches_oracle(dtype, nwg):
         assert np.array_equal(g.view(ui), r.view(ui))
 
 
+OFFSET_CASES = manifest.stream_offset_cases()
+
+
+@pytest.mark.parametrize("case", OFFSET_CASES, ids=[c[0].key() for c in OFFSET_CASES])
+def test_emulated_offset_order_stream_kernel_matches_oracle(case):
+    """Offset-major statements in a 2D STREAM kernel (each weight used by every output at once,
+    rows loaded and x-neighbours lane-shifted where first needed): bit-exact against the oracle
+    after 2 steps, which sums each output's terms in the reference's order."""
+    d, size, blocks = case
+    geo = spec.geometry(d, size)
+    og = oracle.Geometry(2, d.range, d.alignment, d.cl_size, size, (1, 1, 1) if d.remainder else d.coarsening)
+    a = np.zeros(og.n_points)
+    oracle.init_seeded(og, 1, 5, a, halo=0.375)
+    mask = oracle.mask_seeded(7, d.n)
+    dt = np.float32 if d.dtype == F32C else np.float64
+    a = a.astype(dt)
+    run = oracle.run_f32c if d.dtype == F32C else oracle.run_f64
+    ref = run(d, og, mask, a.copy(), a.copy(), 2)
+    ui = np.uint32 if dt == np.float32 else np.uint64
+    for block in blocks:
+        nblk = tuple((geo.threads[i] + block[i] - 1) // block[i] for i in range(3))
+        got, bad = emulate.run(d, geo, mask.astype(dt), a.copy(), a.copy(), 2, block, nblk)
+        assert bad == 0
+        diff = np.flatnonzero(got.view(ui) != ref.view(ui))
+        assert diff.size == 0, f"block {block}: {diff.size} points differ, first {diff[:5]}"
+
+
 @pytest.mark.parametrize("dtype", ["f64", "f32c"])
 def test_emulated_packed_batched_kernel_matches_oracle(dtype):
     """Packed tiles (runs of block x * block y consecutive threads of a grid's

@@ -12,7 +12,8 @@ shape (no reference counterpart) are given as --name=value:
 --variant=plain|stream|temporal, --xext=load|dpp|dppw, --dtype=f64|f32c,
 --tsteps=N (temporal variant, 2D: N time steps per launch through LDS),
 --max-threads=N, --prefetch=N, --waves-per-eu=N, --comp-group=N,
---order=point|offset (plain variant), --no-xcd-remap, --no-nt-store, --zfirst,
+--order=point|offset (plain variant; 2D stream kernels without --prefetch), --no-xcd-remap,
+--no-nt-store, --zfirst,
 --remainder (grids whose interior is not a multiple of the coarsening are
 updated whole instead of truncated), --edge-pairs (--xext=dppw: the wave-edge
 lanes fetch two adjacent rows' extra columns per load), --stage=BXxBYxBZ (3D,
@@ -122,7 +123,11 @@ def desc_from_argv(dim: int, argv):
         if "order" in opts:
             _err("--order applies to the plain variant", usage)
     elif "order" in opts:
-        _err("--order applies to the plain variant", usage)
+        if dim != 2:
+            _err("--order applies to the plain variant and to 2D stream kernels", usage)
+        kw["order"] = opts["order"]
+        if opts["order"] == "offset" and "prefetch" not in opts:
+            kw["prefetch"] = 0
     try:
         return spec.StencilDesc(**kw), out
     except ValueError as e:

@@ -870,6 +870,91 @@ class _Emitter:
                     # (thread rows are whole waves: the condition is uniform across each wave)
                     w(f"  if (tcsg_uni(threadIdx.y) + {i * BY}u < {NROW}u) {st}" if (i + 1) * BY > NROW else f"  {st}")
 
+        def dpp_value(p):
+            """Emit the lane-shift chain that brings x-neighbour p into this lane (main chain
+            from the own columns, halo chain from the edge lanes' extra loads)."""
+            px, py, pz = p
+            nm = vname(px, py, pz)
+            if 0 <= px < XC or nm in declared:
+                return
+            side, h, ip = hop(px)
+            main_fn, e_fn = ("tcsg_shr1", "tcsg_shl1") if side == "L" else ("tcsg_shl1", "tcsg_shr1")
+            gname = lambda j: ename(side, j, ip, py, pz)
+            if (py, pz) in pair_role and pair_role[(py, pz)][0] == "b" and gname(0) not in declared:
+                pya = pair_role[(py, pz)][1]
+                declared.add(gname(0))
+                w(f"  const TV {gname(0)} = tcsg_row{'shl' if side == 'L' else 'shr'}{H}("
+                  f"eP_{ip}_{_n(pya)}_{_n(pz)});")
+            for j in range(1, H):
+                if gname(j) not in declared:
+                    declared.add(gname(j))
+                    w(f"  const TV {gname(j)} = {e_fn}({gname(j - 1)});")
+            if wave_aligned:
+                prev = vname(ip, py, pz)
+                for hh in range(1, h + 1):
+                    sn = f"m{side}{hh}_{ip}_{_n(py)}_{_n(pz)}"
+                    if sn not in declared:
+                        declared.add(sn)
+                        w(f"  const TV {sn} = {main_fn}_or({gname(H - hh)}, {prev});")
+                    prev = sn
+                w(f"  const TV {nm} = {prev};")
+            else:
+                prev = vname(ip, py, pz)
+                for hh in range(1, h + 1):
+                    sn = f"s{side}{hh}_{ip}_{_n(py)}_{_n(pz)}"
+                    if sn not in declared:
+                        declared.add(sn)
+                        w(f"  const TV {sn} = {main_fn}({prev});")
+                    prev = sn
+                w(f"  const TV {nm} = tcsg_sel(t{side} >= {h}u, {prev}, {gname(H - h)});")
+            declared.add(nm)
+
+        def store_output(o, final_of_row):
+            """Store output o; with remainder column groups, the row's wide stores once its last
+            column is final (final_of_row)."""
+            i, j, kk = o
+            if not xgroups:
+                rs, so = st_rs(j, kk)
+                w(f"  tcsg_st({rs}, sbase, {so}, {(i + XL) * es}u, {self.final_expr(o)});")
+                return
+            w(f"  const TV f_{i}_{j}_{kk} = {self.final_expr(o)};")
+            if final_of_row:
+                row = [f"f_{c}_{j}_{kk}" for c in range(XC)]
+                rs, so = st_rs(j, kk)
+                # wide stores of 2 or 4 elements, at most 16 bytes each
+                c0 = 0
+                while c0 < XC:
+                    n = max(m for m in (1, 2, 4) if m * es <= 16 and c0 + m <= XC)
+                    part = row[c0:c0 + n]
+                    if n == 1:
+                        w(f"  tcsg_st({rs}, sfull, {so}, {c0 * es}u, {part[0]});")
+                    else:
+                        w(f"  tcsg_st{n}({rs}, sfull, {so}, {c0 * es}u, {', '.join(part)});")
+                    c0 += n
+                for c in range(XC - 1):
+                    w(f"  tcsg_st({rs}, spart{c}, {so}, 0u, {row[c]});")
+
+        if d.order == "offset":
+            # offset-major statements (2D): for each stencil offset in summation order, the term
+            # of every output, so each weight is used by all outputs at once and streams through
+            # the scalar registers; each output's own sum keeps the reference's order.  A row's
+            # loads are issued where the row is first needed, its x-neighbours lane-shifted
+            # where first needed
+            touched = set()
+            for t, (dx, dy, dz) in enumerate(self.offs):
+                m = spec.mask_index(d, dx, dy, dz)
+                for o in self.outputs():
+                    p = (o[0] + dx, o[1] + dy, o[2] + dz)
+                    if skey(p) not in touched:
+                        touched.add(skey(p))
+                        emit_loads(skey(p))
+                    if dpp:
+                        dpp_value(p)
+                    w("  " + self.acc_stmt(o, t, m, vname(*p)))
+            for o in self.outputs():
+                store_output(o, o[0] == XC - 1)
+            return
+
         D = d.prefetch
         if staged:
             assert order == sorted(order) and D == 0
@@ -886,43 +971,8 @@ class _Emitter:
             emit_loads(k, gi % 2)
             gpts = [p for _, p in members[k]]
             if dpp:
-                # x-neighbours: main lane-shift chain from the own columns, halo
-                # chain from the edge lanes' extra loads
-                for (px, py, pz) in gpts:
-                    nm = vname(px, py, pz)
-                    if 0 <= px < XC or nm in declared:
-                        continue
-                    side, h, ip = hop(px)
-                    main_fn, e_fn = ("tcsg_shr1", "tcsg_shl1") if side == "L" else ("tcsg_shl1", "tcsg_shr1")
-                    gname = lambda j: ename(side, j, ip, py, pz)
-                    if (py, pz) in pair_role and pair_role[(py, pz)][0] == "b" and gname(0) not in declared:
-                        pya = pair_role[(py, pz)][1]
-                        declared.add(gname(0))
-                        w(f"  const TV {gname(0)} = tcsg_row{'shl' if side == 'L' else 'shr'}{H}("
-                          f"eP_{ip}_{_n(pya)}_{_n(pz)});")
-                    for j in range(1, H):
-                        if gname(j) not in declared:
-                            declared.add(gname(j))
-                            w(f"  const TV {gname(j)} = {e_fn}({gname(j - 1)});")
-                    if wave_aligned:
-                        prev = vname(ip, py, pz)
-                        for hh in range(1, h + 1):
-                            sn = f"m{side}{hh}_{ip}_{_n(py)}_{_n(pz)}"
-                            if sn not in declared:
-                                declared.add(sn)
-                                w(f"  const TV {sn} = {main_fn}_or({gname(H - hh)}, {prev});")
-                            prev = sn
-                        w(f"  const TV {nm} = {prev};")
-                    else:
-                        prev = vname(ip, py, pz)
-                        for hh in range(1, h + 1):
-                            sn = f"s{side}{hh}_{ip}_{_n(py)}_{_n(pz)}"
-                            if sn not in declared:
-                                declared.add(sn)
-                                w(f"  const TV {sn} = {main_fn}({prev});")
-                            prev = sn
-                        w(f"  const TV {nm} = tcsg_sel(t{side} >= {h}u, {prev}, {gname(H - h)});")
-                    declared.add(nm)
+                for p in gpts:
+                    dpp_value(p)
             for idx, p in members[k]:
                 v = vname(*p)
                 for (o, t, m) in sorted(contrib[p], key=lambda c: c[0]):
@@ -931,26 +981,7 @@ class _Emitter:
                     if last[o] != idx:
                         continue
                     i, j, kk = o
-                    if not xgroups:
-                        rs, so = st_rs(j, kk)
-                        w(f"  tcsg_st({rs}, sbase, {so}, {(i + XL) * es}u, {self.final_expr(o)});")
-                        continue
-                    w(f"  const TV f_{i}_{j}_{kk} = {self.final_expr(o)};")
-                    if idx == max(last[(c, j, kk)] for c in range(XC)):
-                        row = [f"f_{c}_{j}_{kk}" for c in range(XC)]
-                        rs, so = st_rs(j, kk)
-                        # wide stores of 2 or 4 elements, at most 16 bytes each
-                        c0 = 0
-                        while c0 < XC:
-                            n = max(m for m in (1, 2, 4) if m * es <= 16 and c0 + m <= XC)
-                            part = row[c0:c0 + n]
-                            if n == 1:
-                                w(f"  tcsg_st({rs}, sfull, {so}, {c0 * es}u, {part[0]});")
-                            else:
-                                w(f"  tcsg_st{n}({rs}, sfull, {so}, {c0 * es}u, {', '.join(part)});")
-                            c0 += n
-                        for c in range(XC - 1):
-                            w(f"  tcsg_st({rs}, spart{c}, {so}, 0u, {row[c]});")
+                    store_output(o, not xgroups or idx == max(last[(c, j, kk)] for c in range(XC)))
             if staged and gi + 1 < len(order):
                 stage_writes(gi + 1)
                 w("  __syncthreads();")

@@ -197,6 +197,28 @@ def cfg4_desc(dtype=F64, shape="dpp", batched=False, max_threads=256):
     return d.replace(batched=batched) if batched else d
 
 
+def stream_offset_cases():
+    """(desc, grid size incl. halo, blocks): 2D STREAM kernels with offset-major statements, every
+    x-neighbour exchange mode, both neighbourhoods and orientations, Variable and Jacobi, FP64 and
+    FP32C, remainder handling, and the 2D Box r=8 coarsening with paired edge loads."""
+    base = dict(dimension=2, variant=STREAM, order="offset", prefetch=0, xcd_remap=True, nt_store=True)
+    out = []
+    for xext, blocks in (("dppw", [(64, 2, 1), (128, 1, 1)]), ("dpp", [(16, 4, 1), (96, 2, 1)]),
+                         ("load", [(16, 4, 1)])):
+        out += [
+            (StencilDesc(neighbourhood=BOX, orientation=ROW, coeff=VARIABLE, range=3, xc=2, yc=3, alignment=ALIGN_READ,
+                         cl_size=8, xext=xext, **base), (45, 23), blocks),
+            (StencilDesc(neighbourhood=BOX, orientation=COLUMN, coeff=JACOBI, range=2, xc=4, yc=2, alignment=ALIGN_NONE,
+                         cl_size=0, xext=xext, remainder=True, **base), (43, 21), blocks),
+            (StencilDesc(neighbourhood=STAR, orientation=ROW, coeff=VARIABLE, range=4, xc=2, yc=2,
+                         alignment=ALIGN_WRITE, cl_size=16, dtype=F32C, comp_group=5, xext=xext, **base), (52, 20),
+             blocks),
+        ]
+    out.append((StencilDesc(neighbourhood=BOX, orientation=ROW, coeff=VARIABLE, range=8, xc=4, yc=4, alignment=ALIGN_READ,
+                            cl_size=16, xext="dppw", edge_pairs=True, **base), (96, 36), [(64, 1, 1)]))
+    return out
+
+
 def packed_star_desc(dtype=F64):
     """A batched 3D Star Variable r=3 kernel over packed tiles (x-neighbours by loads, remainder
     handling, Write alignment): the weights and a wider footprint through the packed tile map."""
@@ -290,6 +312,7 @@ def default_build_set():
     descs += [cfg4_desc(dt, shape, batched=True, max_threads=1024 if shape == "plain" else 256).replace(
         remainder=True, pack=True) for dt in (F64, F32C) for shape in ("plain", "stream")]
     descs += [packed_star_desc(dt) for dt in (F64, F32C)]
+    descs += [d for d, _, _ in stream_offset_cases()]
     descs += [c[1] for c in store_hazard_cases()]
     descs += [d for d, _, _ in staged_cases()] + [d for d, _, _ in staged_remainder_cases()]
     from . import tuned, tuner

@@ -78,10 +78,10 @@ class StencilDesc:
     edge_pairs: bool = False  # dppw: the edge columns of two adjacent rows in one load (the
     #                          second row's in the next edge lanes, moved by a DPP row shift)
     tsteps: int = 1          # TEMPORAL: time steps per launch
-    order: str = "point"     # PLAIN statement order: "point" loads each footprint point once and
-    #                          adds it to every output that uses it; "offset" walks the mask in
-    #                          summation order, every output per offset (weights used in order,
-    #                          each loaded point kept until its last use)
+    order: str = "point"     # statement order: "point" loads each footprint point once and adds it
+    #                          to every output that uses it; "offset" (PLAIN, and 2D STREAM without
+    #                          prefetch) walks the mask in summation order, every output per offset
+    #                          (weights used in order, each loaded point kept until its last use)
     stage: tuple = (0, 0, 0)  # STREAM 3D: LDS halo staging for the block shape (bx, by, bz) the
     #                          kernel is then launched with: each z-slice of the block loads the
     #                          rows of every plane once, with the y halo, into LDS (two buffers),
@@ -124,8 +124,10 @@ class StencilDesc:
             raise ValueError("comp_group must be >= 1")
         if self.order not in ("point", "offset"):
             raise ValueError("order must be 'point' or 'offset'")
-        if self.order == "offset" and self.variant != PLAIN:
-            raise ValueError("the offset order is a PLAIN statement order")
+        if self.order == "offset" and self.variant != PLAIN and not (
+                self.variant == STREAM and self.dimension == 2 and not self.staged and self.prefetch == 0):
+            raise ValueError("the offset order is a statement order of PLAIN kernels and of 2D STREAM "
+                             "kernels without LDS staging or register prefetch")
         if self.edge_pairs and (self.variant != STREAM or self.xext != "dppw"):
             raise ValueError("edge_pairs is an option of the wave-aligned DPP code shape")
         if self.variant not in (PLAIN, STREAM, TEMPORAL):

@@ -693,6 +693,59 @@ def main():
     print(json.dumps(line), flush=True)
 
 
+def cpu_rate(desc, sizes, budget_s: float, max_steps: int = 50):
+    """GLup/s of the OpenMP C oracle on this host over grids of `sizes` (incl. halo), same kind
+    of seeded inputs: whole steps of every grid, as many as fit in about budget_s (at least
+    one).  Returns (GLup/s, steps, seconds)."""
+    import numpy as np
+
+    import oracle
+    grids = []
+    for i, size in enumerate(sizes):
+        g = oracle.Geometry(desc.dimension, desc.range, desc.alignment, desc.cl_size, size, desc.coarsening)
+        a = np.empty(g.n_points)
+        oracle.init_seeded(g, 1, INPUT_SEED + i, a)
+        grids.append((g, a, a.copy()))
+    mask = oracle.mask_seeded(MASK_SEED, desc.n)
+    pts = sum(g.interior_points for g, _, _ in grids)
+    t0 = time.perf_counter()
+    for g, a, b in grids:
+        oracle.step_f64(desc, g, mask, a, b)
+    one = time.perf_counter() - t0
+    steps = max(1, min(max_steps, int(budget_s / max(one, 1e-3))))
+    t0 = time.perf_counter()
+    for g, a, b in grids:
+        oracle.run_f64(desc, g, mask, a, b, steps)
+    dt = time.perf_counter() - t0
+    return pts * steps / dt / 1e9, steps, dt
+
+
+def other_cpu_baseline(name, budget_s: float = 3.0):
+    """The oracle's rate for another measured workload's stencil, on its grid or a bounded
+    part of it (FP64; the same per-point work as the GPU run's)."""
+    import oracle
+    from threadcoarsenedstencilgenerator_amd import manifest, tuned
+    from threadcoarsenedstencilgenerator_amd.runtime import batch_sizes
+    if name in ("star3d_r4_large", "star3d_r4_511_remainder"):
+        return {"same_as": "headline cpu_baseline (same stencil, per-point work independent of the grid)"}
+    if name == "box2d_r1_temporal":
+        d, sizes, what = manifest.cfg1_desc("plain"), [tuned.BOX2D_R1_GRID], "2D Box Jacobi r=1 16384^2"
+    elif name == "star3d_r1_temporal":
+        d, sizes, what = tuned.STAR3D_R1_ONE_STEP["desc"], [tuned.STAR3D_R1_GRID], "3D Star Variable r=1 512^3"
+    elif name == "box2d_r8":
+        # a quarter of the rows: one step of the whole grid is ~77 GFLOP
+        d, sizes, what = manifest.cfg3_desc(1, 1), [(16384 + 16, 2048 + 16)], \
+            "2D Box Variable r=8, 16384x2048 (a quarter of the 16384x8192 grid)"
+    elif name == "batched":
+        d = tuned.BATCHED_PLAIN_F64["desc"].replace(batched=False)
+        sizes, what = batch_sizes(64, 1, 32, 256, tuned.BATCH_ROUNDING, d.range), "the 64 uneven 3D Box Jacobi r=2 grids"
+    else:
+        return None
+    rate, steps, dt = cpu_rate(d, sizes, budget_s)
+    return {"value": round(rate, 4), "unit": "GLup/s", "cores": oracle.num_threads(), "kind": "port",
+            "sample": f"{steps} FP64 step(s) of {what} ({dt:.1f} s), OpenMP C oracle"}
+
+
 def other_workloads(args, others):
     """The other measured workloads, each with its own fixed step count, summarised
     (GLup/s, roofline fraction and what each compares against); a workload that
@@ -719,6 +772,8 @@ def other_workloads(args, others):
             summ["vs_one_step"] = ln["vs_one_step"]
             summ["one_step_glups"] = ln["runs"]["one_step"]["glups"]
             summ["kernel_glups"] = ln["runs"]["temporal"]["kernel_glups"]
+        if not args.no_cpu_baseline:
+            summ["cpu_baseline"] = other_cpu_baseline(name)
         if name == "batched":
             d = ln["detail"]
             summ["f32c"] = d["f32c"]["glups"]["persistent_graph"]

@@ -289,13 +289,17 @@ class _Emitter:
                 w(f"__device__ __forceinline__ {T} tcsg_ld(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff) {{")
                 w(f"  return __builtin_bit_cast({T}, __builtin_amdgcn_raw_buffer_load_b{bits}(r, off, soff, {laux}));")
                 w("}")
-                # nontemporal stores (gfx950 cache policy nt = 2), at device scope (sc1 = 16) where a
-                # lane stores at most 16 bytes per row, so one store instruction writes whole lines
-                # of a row: the L2 then writes the lines through instead of holding them dirty (0.5%
-                # faster on the 512^3 Star r=4 kernels than nt alone; sc1 without nt 3% slower).
-                # Wider lanes store a row in several instructions, each writing part of every line,
-                # and write-through then costs (a batched 4-column kernel: 42% slower)
-                aux = (18 if (4 if self.f32 else 8) * self.XC <= 16 else 2) if d.nt_store else 0
+                # nontemporal stores (gfx950 cache policy nt = 2), at device scope (sc1 = 16) where
+                # every store instruction writes whole 128-byte lines once: Write alignment at a
+                # multiple of 128 bytes (interior rows start on a line), at most 16 bytes per lane
+                # and row (one instruction per row), and no remainder handling (whose shifted last
+                # tiles write lines their neighbours also write).  The L2 then writes the lines
+                # through instead of holding them dirty: 0.3-0.5% faster on the 512^3 Star r=4
+                # kernels; where instructions write parts of lines, write-through costs 6-42%
+                es_ = 4 if self.f32 else 8
+                whole_lines = (d.alignment == ALIGN_WRITE and (d.cl_size * es_) % 128 == 0 and not d.remainder
+                               and es_ * self.XC <= 16)
+                aux = (18 if whole_lines else 2) if d.nt_store else 0
                 # a row's columns merge into one store of es*XC bytes (explicit below with
                 # remainder handling); only stores wider than 64 bits need the soffset-0 form
                 wide = (4 if self.f32 else 8) * self.XC > 8

@@ -1666,6 +1666,8 @@ class _Emitter:
             w("  if (leader) tcsg_slot[it & 1u] = tcsg_fetch(ba.queue, ba.ntiles, xcd, nr, s);")
             w("  __syncthreads();")
             if d.variant == PLAIN:
+                # (the PLAIN body builds no buffer resources; taking its tile as wave-uniform
+                # measured 15-21% slower on the 64-grid batch)
                 w("  const uint32_t t = tcsg_slot[it & 1u];")
             else:
                 w("  // the same tile for every lane, taken as wave-uniform: the buffer resources built")

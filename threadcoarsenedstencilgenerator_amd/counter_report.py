@@ -21,6 +21,12 @@ figures that tell latency-, VALU- and TA-bound kernels apart.
 
     python -m threadcoarsenedstencilgenerator_amd.counter_report --meta M.json --trace TRACE_DIR \
         PASS_DIR [PASS_DIR ...] [--md OUT.md] [--json OUT.json]
+
+With --tsv it instead appends counter columns to the `bench` harness's TSV (the reference's
+printResults rows): the passes are `rocprofv3 --pmc ... --kernel-trace` runs of the same
+harness command, whose kernel traces give each dispatch's block shape.
+
+    python -m threadcoarsenedstencilgenerator_amd.counter_report --tsv BENCH.tsv PASS_DIR [PASS_DIR ...]
 """
 from __future__ import annotations
 
@@ -225,16 +231,111 @@ def markdown(rep) -> str:
     return "\n".join(out) + "\n"
 
 
+TSV_COLUMNS = ("HBM [B/Lup]", "HBM [GB/s]", "FP64 [Flop/Lup] counted", "FP64 counted [Flop/s]", "L1 [B/Flop]",
+               "LDS [B/Flop]")
+
+
+def shape_counters(pass_dirs, symbol: str = "tcsg_update"):
+    """{(bx, by, bz): {counter: mean per dispatch}} over the dispatches of `symbol` in counter
+    passes each recorded with a kernel trace of the same run (`rocprofv3 --pmc ... --kernel-trace`):
+    the trace gives every dispatch's block shape, the counter rows only its thread count."""
+    out = defaultdict(lambda: defaultdict(list))
+    for d in pass_dirs:
+        shape = {}
+        for row in _rows(d, "*kernel_trace.csv"):
+            if row["Kernel_Name"] == symbol:
+                shape[row["Dispatch_Id"]] = (int(row["Workgroup_Size_X"]), int(row["Workgroup_Size_Y"]),
+                                             int(row["Workgroup_Size_Z"]))
+        per = defaultdict(lambda: defaultdict(float))
+        for row in _rows(d, "*counter_collection.csv"):
+            if row["Kernel_Name"] == symbol and row["Dispatch_Id"] in shape:
+                per[row["Dispatch_Id"]][row["Counter_Name"]] += float(row["Counter_Value"])
+        for did, cs in per.items():
+            for c, v in cs.items():
+                out[shape[did]][c].append(v)
+    return {b: {c: sum(v) / len(v) for c, v in cs.items()} for b, cs in out.items()}
+
+
+def tsv_with_counters(tsv_text: str, counters, fetch_scale: float, write_scale: float,
+                      steps_per_launch: int = 1) -> str:
+    """The harness's TSV (the reference's printResults layout, hip/benchmark.cpp:352-426) with
+    counter columns appended to every row whose block shape the passes measured: HBM bytes per
+    lattice update (FETCH_SIZE and WRITE_SIZE in KiB, scaled by what they read for a copy of known
+    size) and the GB/s they make at the row's Lup/s (the HIP-event column where the TSV has one,
+    else the mean of the reference's protocol), FP64 flops per update as counted
+    (SQ_INSTS_VALU_FLOPS_FP64 x 64 lanes) and the Flop/s they make, and L1 (TCP tag accesses x
+    128 B) and LDS (SQ_INSTS_LDS_*_BANDWIDTH x 64 B) bytes per counted flop; '-' where a counter
+    is missing.  Rows of shapes the passes did not see are left as they are."""
+    lines = tsv_text.rstrip("\n").split("\n")
+    pts = None
+    for ln in lines:
+        if ln.startswith("#grid size excluding border:"):
+            v = ln.split(":", 1)[1].split()
+            dims = [int(v[i + 1]) for i in range(0, len(v), 2)]
+            pts = 1
+            for x in dims:
+                pts *= x
+    if pts is None:
+        raise ValueError("no '#grid size excluding border' line in the TSV")
+    upd = pts * steps_per_launch
+    out = []
+    rate_col = 5   # Perf [Lup/s]; the HIP-event rate instead where the TSV has it (TCSG_EXTRA_COLUMNS)
+    for ln in lines:
+        if ln.startswith("#xthreads"):
+            cols = ln.lstrip("#").split("\t")
+            if "event Perf [Lup/s]" in cols:
+                rate_col = cols.index("event Perf [Lup/s]")
+            out.append(ln + "".join("\t" + c for c in TSV_COLUMNS))
+            continue
+        if ln.startswith("#") or not ln.strip():
+            out.append(ln)
+            continue
+        f = ln.split("\t")
+        b = (int(f[0]), int(f[1]), max(1, int(f[2])))
+        k = counters.get(b)
+        if not k:
+            out.append(ln)
+            continue
+        lups = float(f[rate_col])
+        g = k.get
+        fmt = lambda v: "-" if v is None else f"{v:.6e}"
+        hbm = (g("FETCH_SIZE") * 1024 * fetch_scale + g("WRITE_SIZE") * 1024 * write_scale) / upd \
+            if g("FETCH_SIZE") is not None and g("WRITE_SIZE") is not None else None
+        fl = g("SQ_INSTS_VALU_FLOPS_FP64") * WAVE / upd if g("SQ_INSTS_VALU_FLOPS_FP64") else None
+        l1 = g("TCP_TOTAL_CACHE_ACCESSES_sum") * L1_LINE / upd if g("TCP_TOTAL_CACHE_ACCESSES_sum") is not None else None
+        lds = ((g("SQ_INSTS_LDS_LOAD_BANDWIDTH") or 0) + (g("SQ_INSTS_LDS_STORE_BANDWIDTH") or 0)) * LDS_UNIT / upd \
+            if g("SQ_INSTS_LDS_LOAD_BANDWIDTH") is not None else None
+        vals = (hbm, hbm * lups / 1e9 if hbm is not None else None, fl, fl * lups if fl else None,
+                l1 / fl if l1 is not None and fl else None, lds / fl if lds is not None and fl else None)
+        out.append(ln + "".join("\t" + fmt(v) for v in vals))
+    return "\n".join(out) + "\n"
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("passes", nargs="+")
-    ap.add_argument("--meta", required=True)
-    ap.add_argument("--trace", required=True)
+    ap.add_argument("--tsv", help="instead: append counter columns to this harness TSV; the passes are "
+                                  "`rocprofv3 --pmc ... --kernel-trace` runs of the same harness command")
+    ap.add_argument("--fetch-scale", type=float, default=2.0, help="--tsv: bytes per FETCH_SIZE KiB unit, "
+                    "as the 2^30-byte calibration copy reads it")
+    ap.add_argument("--write-scale", type=float, default=1.0)
+    ap.add_argument("--steps-per-launch", type=int, default=1)
+    ap.add_argument("--meta")
+    ap.add_argument("--trace")
     ap.add_argument("--md")
     ap.add_argument("--json")
     ap.add_argument("--traffic", help="update this JSON (kernel key -> HBM bytes per launch, as bench.py "
                                       "reads it) with the workloads' FETCH_SIZE/WRITE_SIZE traffic")
     a = ap.parse_args()
+    if a.tsv:
+        text = tsv_with_counters(open(a.tsv).read(), shape_counters(a.passes), a.fetch_scale, a.write_scale,
+                                 a.steps_per_launch)
+        print(text, end="")
+        if a.md:
+            open(a.md, "w").write(text)
+        return
+    if not a.meta or not a.trace:
+        ap.error("--meta and --trace are required without --tsv")
     meta = json.load(open(a.meta))
     rep = report(meta, [read_pass(p, meta) for p in a.passes], read_trace(a.trace, meta))
     md = markdown(rep)

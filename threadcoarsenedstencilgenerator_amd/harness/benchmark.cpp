@@ -74,6 +74,8 @@ void StencilBenchmark::init_gpu()
     if (hipGetDeviceProperties(&p, 0) != hipSuccess) die("hipGetDeviceProperties failed.");
     for (int a = 0; a < 3; ++a) max_dims_[a] = p.maxThreadsDim[a];
     gpu_name_ = p.name;
+    // the marketing name can be empty where the driver's name table is missing: the ISA name then
+    if (gpu_name_.empty()) gpu_name_ = p.gcnArchName;
     warp_ = p.warpSize;
 }
 

@@ -108,6 +108,10 @@ def batched_candidates(dtype):
         for cg in (13, 25):
             out.append((f"plain 2x2x4 group{cg}", plain.replace(comp_group=cg), (32, 8, 2)))
             out.append((f"load 2x4x16 pack group{cg}", d.replace(comp_group=cg), (32, 8, 1)))
+        # with groups of 25 (fewer live registers): four columns per lane
+        for co in ((4, 4, 8), (4, 4, 12), (4, 4, 16), (2, 8, 16), (4, 2, 16)):
+            out.append((f"load {co[0]}x{co[1]}x{co[2]} pack group25",
+                        d.replace(xc=co[0], yc=co[1], zc=co[2], comp_group=25), (16, 16, 1)))
     return out
 
 

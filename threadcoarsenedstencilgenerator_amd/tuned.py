@@ -109,11 +109,13 @@ STAR3D_R1_ONE_STEP = dict(desc=StencilDesc(dimension=3, neighbourhood=STAR, orie
 BATCH_ROUNDING = (2, 2, 4)
 BATCHED_F64 = dict(desc=manifest.cfg4_desc(shape="stream", batched=True, max_threads=256).replace(
     xc=2, yc=4, zc=16, remainder=True, zfirst=False, pack=True), block=(32, 8, 1))
-# FP32C: TwoSum groups of 25 terms, one z plane of the 5x5x5 box each (218.2 GLup/s in the sweep
-# against 202.8 for groups of 5; normwise error against the FP64 kernel after 20 steps 1.20e-6
-# against 1.07e-6, both set by the FP32 rounding of the stored U[0,1) state at every step)
+# FP32C: TwoSum groups of 25 terms, one z plane of the 5x5x5 box each (fewer live error terms: 108
+# VGPRs at 2x4x16 against 208 for groups of 5, 218.2 GLup/s in the sweep against 202.8), which leaves
+# room for 4 columns per lane: 4x4x8 over packed runs of 256 threads, 305.4 GLup/s in the sweep
+# (4x4x12 304.6, 4x4x16 287.3, 2x8x16 271.2; 315.1 in the bench line).  Normwise error against the FP64 kernel after 20 steps 1.2e-6 (groups of 5:
+# 1.07e-6), set by the FP32 rounding of the stored U[0,1) state at every step.
 BATCHED_F32C = dict(desc=manifest.cfg4_desc(shape="stream", batched=True, dtype=F32C, max_threads=256).replace(
-    xc=2, yc=4, zc=16, remainder=True, zfirst=False, nt_store=False, pack=True, comp_group=25), block=(32, 8, 1))
+    xc=4, yc=4, zc=8, remainder=True, zfirst=False, nt_store=False, pack=True, comp_group=25), block=(16, 16, 1))
 # the tuned kernels before packing: the reference's code shape (PLAIN) with a z chunk of 8 planes
 # per thread (12 planes loaded for 8 outputs instead of 8 for 4) over rectangular tiles; FP64 2x2x8
 # at 32x8x2, FP32C 1x2x8 at 32x8x2
